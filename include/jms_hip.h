@@ -1,0 +1,115 @@
+/* jms_hip.h -- thin C-ABI between the host-side CJMultiSlice class and the
+ * HIP kernels of the multislice slice loop (MI355X / gfx950).
+ *
+ * All functions return 0 on success and >0 on failure, with GPU-side failures
+ * in the 1xx range (the reference's convention, JMultiSlice.cpp:499,566,762-800).
+ * Plain pointers and sizes only; complex arrays are interleaved complex64
+ * (re, im float pairs), row-major with x fastest (index x + y*nx), Fourier-space
+ * arrays in FFT ("scrambled") order -- the reference's fcmplx layout
+ * (fcomplex.h:27, JMultiSlice.cpp:1036-1041).
+ *
+ * An engine owns one HIP stream on the device that is current when it is
+ * created.  Calls on one engine must not overlap; distinct engines are
+ * independent.
+ */
+#ifndef JMS_HIP_H
+#define JMS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jms_hip_engine jms_hip_engine;
+
+/* detection channel flags (JMultiSlice.h:141-150) */
+#define JMS_HIP_DETECT_INTEGRATED 1
+#define JMS_HIP_DETECT_DIFFRACTION 4
+#define JMS_HIP_DETECT_WAVEFOURIER 16
+
+/* engine lifetime ---------------------------------------------------------- */
+int jms_hip_create(jms_hip_engine** out);
+void jms_hip_destroy(jms_hip_engine* e);
+/* 1 when nx (row length) and ny (column length) have compiled kernels */
+int jms_hip_grid_supported(int nx, int ny);
+
+/* setup (replaces the device side of the reference setup chain,
+ * JMultiSlice.h:76-90) --------------------------------------------------- */
+/* grid size, super-cell size [nm] and electron wavelength [nm] */
+int jms_hip_set_grid(jms_hip_engine* e, int nx, int ny, float a0x, float a0y, float wl);
+/* phase-grating store: nslc structure slices with nvarslc[i] variants each,
+ * gratings of npgx*npgy (must equal the grid) -- PhaseGratingSetup :447 */
+int jms_hip_setup_gratings(jms_hip_engine* e, int npgx, int npgy, int nslc, const int* nvarslc);
+/* upload nvar variants of structure slice islc -- SetPhaseGratingData :1440 */
+int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* pgr);
+/* npro propagators, proidx[nslc] selects one per structure slice -- :542 */
+int jms_hip_setup_propagators(jms_hip_engine* e, int npro, int nslc, const int* proidx);
+int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro);
+/* object slice sequence (structure slice index per object slice) -- :522 */
+int jms_hip_setup_objects(jms_hip_engine* e, int nobjslc, const int* objslc);
+/* detection: det_objslc[nobjslc+1] maps planes to detection indices
+ * (-1 = none), ndet integrating detectors, flags = JMS_HIP_DETECT_* -- :668 */
+int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objslc, int ndet,
+                            int flags);
+/* detector function det[nx*ny]; with msklen>0 only the listed pixel indices
+ * count (each listing once) -- SetDetectorData :1505 */
+int jms_hip_upload_detector(jms_hip_engine* e, int idet, const float* det, int msklen,
+                            const int* msk);
+/* incident (probe) wave in Fourier space -- SetIncidentWave :1805 */
+int jms_hip_set_probe(jms_hip_engine* e, const float* wav0);
+/* constant phase plate [rad] (nx*ny floats), NULL = zero -- :1858-1932 */
+int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl);
+
+/* batched STEM scan -------------------------------------------------------- */
+/* Stage a scan on the device: npos positions (dx, dy, dz in nm; dz may be
+ * NULL), npass passes per position with weights[npass], and the variant
+ * sequences var[(p*npass + r)*nobjslc + j] (position-major, pass-minor).
+ * Results are zeroed. */
+int jms_hip_stage_scan(jms_hip_engine* e, int npos, const float* dx, const float* dy,
+                       const float* dz, int npass, const float* weights, const int* var);
+/* run positions [pos0, pos0+npos) of the staged scan from plane islc0 on
+ * device-resident inputs (asynchronous; jms_hip_sync waits).  Each position
+ * gets sum_r weights[r] * (readout of pass r) on its detection planes. */
+int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0);
+int jms_hip_sync(jms_hip_engine* e);
+/* integrated results of the staged scan: npos*ndetslc*ndet floats,
+ * [pos][idetslc][idet] */
+int jms_hip_get_scan_results(jms_hip_engine* e, float* out);
+/* batch size in probe passes (waves resident at once); 0 = automatic */
+int jms_hip_set_batch(jms_hip_engine* e, int nwave_max);
+/* 1: capture the per-batch launch sequence in a HIP graph and replay it */
+int jms_hip_set_graph(jms_hip_engine* e, int use_graph);
+
+/* single-wave path (CJMultiSlice::GPUMultislice) --------------------------- */
+/* wave <- probe * exp(-i[2pi(kx dx + ky dy + k^2 dz wl/2) + cppl]), with a
+ * cyclic beam-tilt shift (ibtx, ibty) pixels -- OffsetIncomingWave :1961 */
+int jms_hip_wave_offset(jms_hip_engine* e, float dx, float dy, float dz, int ibtx, int ibty,
+                        int use_cppl);
+/* wave <- host data (Fourier space) -- SetIncomingWaveGPU :4468 */
+int jms_hip_wave_set(jms_hip_engine* e, const float* wav);
+/* one multislice run of the wave from plane islc0 with variant sequence
+ * var[nobjslc]; readouts are accumulated with `weight`, after clearing the
+ * channels when accumulate == 0 -- GPUMultislice :4513 */
+int jms_hip_multislice(jms_hip_engine* e, int islc0, const int* var, float weight, int accumulate);
+/* copy a result channel of the single-wave path: which = JMS_HIP_DETECT_*;
+ * INTEGRATED: ndetslc*ndet floats; DIFFRACTION: ndetslc*nx*ny floats;
+ * WAVEFOURIER: 2*ndetslc*nx*ny floats -- GetResult :2406 */
+int jms_hip_get_result(jms_hip_engine* e, int which, float* dst);
+/* copy the current wave (2*nx*ny floats) */
+int jms_hip_wave_get(jms_hip_engine* e, float* dst);
+
+/* utilities ---------------------------------------------------------------- */
+/* in-place 2-D FFT of nbatch host arrays (dir < 0: forward exp(-i), dir > 0:
+ * backward exp(+i), unscaled) on the device -- JFFTCUDAcore::FT/IFT :299-336 */
+int jms_hip_fft2d(int nx, int ny, int nbatch, float* data, int dir);
+/* kernel timing with HIP events on the engine stream (eager launches only):
+ * enable, then read the summed duration [ms] and launch count per kernel kind
+ * (0 = row pass, 1 = column pass, 2 = combine) */
+int jms_hip_set_timing(jms_hip_engine* e, int enable);
+int jms_hip_get_timing(jms_hip_engine* e, int kind, double* ms, long long* count);
+/* last HIP error text of the engine (static storage) */
+const char* jms_hip_last_error(jms_hip_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JMS_HIP_H */

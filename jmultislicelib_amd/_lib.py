@@ -1,0 +1,155 @@
+"""ctypes binding of libjmultislice_hip.so (the MI355X multislice library).
+
+The shared library is built in-tree by ``jmultislicelib_amd/csrc/Makefile``
+(``__graft_entry__.build()``).  Loading fails loudly when it is missing: the
+package has no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libjmultislice_hip.so")
+
+_lib = None
+
+c_int = C.c_int
+c_float = C.c_float
+c_double = C.c_double
+c_void_p = C.c_void_p
+c_char_p = C.c_char_p
+c_longlong = C.c_longlong
+
+# (name, restype, argtypes) of every exported entry point
+_SIGS = [
+    # flat C-ABI (include/JMultiSliceLib.h)
+    ("GetJMSVersion", c_float, []),
+    ("SetJMSDebugLevel", c_int, [c_int]),
+    ("SetRng", None, [c_void_p]),
+    ("GetRng", c_void_p, []),
+    ("SeedRngEx", None, [c_int]),
+    ("SetHighTension", c_float, [c_float]),
+    ("GetHighTension", c_float, []),
+    ("GetWaveLength", c_float, []),
+    ("SetSupercellSize", None, [c_void_p]),
+    ("SetSupercellSizeABC", None, [c_float, c_float, c_float]),
+    ("SetSliceThickness", None, [c_int, c_float]),
+    ("SetGridSize", None, [c_int, c_int]),
+    ("GetGridSize", None, [c_void_p, c_void_p]),
+    ("GetDetetionSliceNum", c_int, []),
+    ("GetDetNum", c_int, []),
+    ("GetCPUNum", c_int, []),
+    ("GetGPUNum", c_int, []),
+    ("GetGPUName", c_int, [c_int, c_char_p]),
+    ("GetCurrentGPU", c_int, []),
+    ("SetCurrentGPU", c_int, [c_int]),
+    ("GetGPUStats", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("GetUnscrambleHash", c_int, [c_void_p]),
+    ("PhaseGratingSetup", c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p]),
+    ("ObjectSliceSetup", c_int, [c_int, c_void_p]),
+    ("PropagatorSetup", c_int, [c_int, c_int, c_void_p]),
+    ("DetectorSetup", c_int, [c_int, c_int, c_int, c_int, c_int]),
+    ("InitCore", c_int, [c_int, c_int]),
+    ("SetIncidentWave", c_int, [c_int, c_void_p, c_int]),
+    ("SetIncidentWavePhaseplate", c_int, [c_int, c_void_p, c_int]),
+    ("ZeroIncidentWavePhaseplate", c_int, [c_int]),
+    ("SetSlicePhaseGratings", c_int, [c_int, c_int, c_int, c_void_p]),
+    ("SetPropagatorData", c_int, [c_int, c_int, c_void_p]),
+    ("SetDetectorData", c_int, [c_int, c_int, c_void_p, c_int, c_void_p]),
+    ("SetGPUPgrLoading", c_int, [c_int]),
+    ("CalculateProbeWaveFourier", c_int, [c_void_p, c_void_p]),
+    ("SetIncomingWaveGPU", c_int, [c_void_p, c_int]),
+    ("CalculatePropagator", c_int, [c_float, c_float, c_float, c_void_p, c_int]),
+    ("CalculateRingDetector", c_int, [c_float, c_float, c_float, c_float, c_float, c_float, c_char_p,
+                                      c_void_p, c_void_p, c_void_p, c_int]),
+    ("OffsetIncomingWave", c_int, [c_int, c_float, c_float, c_float, c_int, c_int, c_int, c_int]),
+    ("GPUMultislice", c_int, [c_int, c_int, c_float]),
+    ("GetResult", c_int, [c_int, c_int, c_void_p, c_int]),
+    ("ClearDetMem_d", c_int, []),
+    ("GPUMultisliceScan", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("GPUMultisliceScanVar", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                     c_void_p]),
+    ("GetRandomVariantSequence", c_int, [c_void_p]),
+    ("Cleanup", c_int, []),
+    ("FreeLibMem", None, []),
+    # thin engine ABI (include/jms_hip.h)
+    ("jms_hip_create", c_int, [c_void_p]),
+    ("jms_hip_destroy", None, [c_void_p]),
+    ("jms_hip_grid_supported", c_int, [c_int, c_int]),
+    ("jms_hip_set_grid", c_int, [c_void_p, c_int, c_int, c_float, c_float, c_float]),
+    ("jms_hip_setup_gratings", c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    ("jms_hip_upload_gratings", c_int, [c_void_p, c_int, c_int, c_void_p]),
+    ("jms_hip_setup_propagators", c_int, [c_void_p, c_int, c_int, c_void_p]),
+    ("jms_hip_upload_propagator", c_int, [c_void_p, c_int, c_void_p]),
+    ("jms_hip_setup_objects", c_int, [c_void_p, c_int, c_void_p]),
+    ("jms_hip_setup_detection", c_int, [c_void_p, c_int, c_void_p, c_int, c_int]),
+    ("jms_hip_upload_detector", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    ("jms_hip_set_probe", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_set_phaseplate", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_stage_scan", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                   c_void_p]),
+    ("jms_hip_run_staged", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("jms_hip_sync", c_int, [c_void_p]),
+    ("jms_hip_get_scan_results", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_set_batch", c_int, [c_void_p, c_int]),
+    ("jms_hip_set_graph", c_int, [c_void_p, c_int]),
+    ("jms_hip_wave_offset", c_int, [c_void_p, c_float, c_float, c_float, c_int, c_int, c_int]),
+    ("jms_hip_wave_set", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_wave_get", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_multislice", c_int, [c_void_p, c_int, c_void_p, c_float, c_int]),
+    ("jms_hip_get_result", c_int, [c_void_p, c_int, c_void_p]),
+    ("jms_hip_fft2d", c_int, [c_int, c_int, c_int, c_void_p, c_int]),
+    ("jms_hip_set_timing", c_int, [c_void_p, c_int]),
+    ("jms_hip_get_timing", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    ("jms_hip_last_error", c_char_p, [c_void_p]),
+    # variant sequencing (include/jms_sequence.h)
+    ("jms_variant_sequences", c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    # specimen builder (include/jms_specimen.h)
+    ("jms_perovskite_sites", c_int, [c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_double, c_int,
+                                     c_int, c_void_p, c_void_p]),
+    ("jms_gaussian_phase_grating", c_int, [c_int, c_int, c_double, c_double, c_int, c_void_p, c_void_p,
+                                           c_double, c_double, c_void_p]),
+]
+
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def lib() -> C.CDLL:
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray | None):
+    """c_void_p of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+class CJProbeParams(C.Structure):
+    """Probe parameters (include/JMultiSliceLib.h, fields of JProbeGen.h:84-122)."""
+    _fields_ = [
+        ("m_wl", c_float), ("m_alpha", c_float), ("m_alpha_rs", c_float),
+        ("m_alpha_x0", c_float), ("m_alpha_y0", c_float), ("m_alpha_asym", c_float),
+        ("m_alpha_adir", c_float), ("m_btx", c_float), ("m_bty", c_float),
+        ("m_source_width", c_float), ("m_source_shape", c_int), ("m_fspread_width", c_float),
+        ("m_fspread_kernel_width", c_float), ("m_fspread_kernel_samples", c_int),
+        ("m_abrr_coeff", C.POINTER(c_float)), ("m_abrr_num", c_int),
+    ]

@@ -1,0 +1,155 @@
+"""Benchmark workloads: synthetic perovskite specimens and STEM scans.
+
+``perovskite_setup`` builds everything the multislice needs for the cubic
+ABO3 recipe of the benchmark configurations (A = Sr Z38 at the cell corners,
+B = Ti Z22 at the body centre, O Z8 at the face centres; two slices of a/2
+per cell; Gaussian projected potentials; Fresnel propagator; ring
+detectors; aberration-free probe) with the library's own host calculators
+(CalculatePropagator, CalculateRingDetector, CalculateProbeWaveFourier via
+the C-ABI) and specimen builder.  Lengths are in nm at the library interface
+(the reference's unit), in Angstrom in the recipe.
+
+Named workloads:
+  * single_probe_256: 256x256 grid, 4x4 cells (15.62 A), 100 slices, one
+    variant per slice type, one probe on an A column;
+  * stem_scan_512: 512x512 grid, 8x8 cells (31.24 A), 8 frozen-phonon
+    variants per slice type (rms 0.08 A, seeds 1000 + 8*type + v), 200 slices,
+    64x64 probe positions over 2x2 cells, 8 passes per position (weights 1/8),
+    variant sequences from one generator seeded 42.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import lib, ptr
+from .jms import JMultiSlice
+
+A_PEROVSKITE = 3.905          # lattice constant [A]
+SIGMA_POT = 0.3               # Gaussian width of the projected potential [A]
+SIG_INT = 0.00729             # interaction constant [1/(V A)] used by the recipe
+Z_A, Z_B, Z_O = 38, 22, 8
+DETECTOR_RANGES = ((0.0, 11.0), (11.0, 22.0), (70.0, 200.0))  # mrad
+
+
+@dataclass
+class Setup:
+    nx: int
+    ny: int
+    a0: tuple                     # super-cell size [nm] (x, y, z)
+    ht: float                     # [keV]
+    wl: float                     # [nm]
+    nvarslc: list
+    gratings: list                # per structure slice: (nvar, ny, nx) complex64
+    propagators: list             # (ny, nx) complex64
+    proidx: list
+    objslc: np.ndarray
+    det_objslc: np.ndarray
+    ndetslc: int
+    detectors: list               # (det (ny, nx) float32, mask int32 or None)
+    probe: np.ndarray             # (ny, nx) complex64, Fourier space
+    flags: int = 1                # detection channels
+    slice_thickness: float = 0.0  # [nm]
+    sites: list = field(default_factory=list)  # per (type, variant): (xy [A], z)
+
+
+def detection_planes(ndetper: int, nobjslc: int) -> tuple[np.ndarray, int]:
+    """Plane -> detection index hash (CJMultiSlice::SetDetectionPlanes,
+    JMultiSlice.cpp:579-627)."""
+    nos = max(0, nobjslc)
+    h = np.full(nos + 1, -1, np.int32)
+    if ndetper <= 0:
+        h[nos] = 0
+        return h, 1
+    if nos == 0:
+        h[0] = 0
+        return h, 1
+    k = 0
+    for i in range(0, nos + 1, ndetper):
+        h[i] = k
+        k += 1
+    if nos % ndetper:
+        h[nos] = k
+        k += 1
+    return h, k
+
+
+def perovskite_sites(ncell: int, slice_type: int, nvar: int, v: int, rms: float, seed_base: int):
+    """Sites (xy [A], z) of one slice type and frozen-phonon variant."""
+    ncs = ncell * ncell * (2 if slice_type == 0 else 3)
+    xy = np.zeros((ncs, 2), np.float64)
+    z = np.zeros(ncs, np.int32)
+    seed = seed_base + 8 * slice_type + v
+    k = lib().jms_perovskite_sites(ncell, ncell, A_PEROVSKITE, slice_type, Z_A, Z_B, Z_O, rms, seed,
+                                   1 if nvar > 1 else 0, ptr(xy), ptr(z))
+    assert k == ncs
+    return xy, z
+
+
+def gaussian_grating(n: int, cell_ang: float, xy: np.ndarray, z: np.ndarray) -> np.ndarray:
+    out = np.zeros((n, n), np.complex64)
+    rc = lib().jms_gaussian_phase_grating(n, n, cell_ang, cell_ang, len(z), ptr(xy), ptr(z), SIGMA_POT,
+                                          SIG_INT, ptr(out))
+    assert rc == 0
+    return out
+
+
+def perovskite_setup(n: int = 512, ncell: int = 8, nslices: int = 200, nvar: int = 8, rms: float = 0.08,
+                     seed_base: int = 1000, ht: float = 200.0, alpha: float = 25.0,
+                     alpha_rs: float = 0.03, ndetper: int = 0, ptype: int = 1,
+                     det_ranges=DETECTOR_RANGES, flags: int = 1) -> Setup:
+    cell_ang = ncell * A_PEROVSKITE
+    a_nm = cell_ang * 0.1
+    dz_nm = 0.5 * A_PEROVSKITE * 0.1
+    J = JMultiSlice()
+    J.SetHighTension(ht)
+    J.SetGridSize(n, n)
+    J.SetSupercellSize(a_nm, a_nm, A_PEROVSKITE * 0.1)
+    wl = J.GetWaveLength()
+    gratings, sites = [], []
+    for t in (0, 1):
+        g = np.zeros((nvar, n, n), np.complex64)
+        for v in range(nvar):
+            xy, z = perovskite_sites(ncell, t, nvar, v, rms, seed_base)
+            sites.append((xy, z))
+            g[v] = gaussian_grating(n, cell_ang, xy, z)
+        gratings.append(g)
+    pro = J.CalculatePropagator(dz_nm, 0.0, 0.0, ptype)  # also selects the detector angle mapping
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in det_ranges]
+    probe = J.CalculateProbeWaveFourier(alpha, alpha_rs)
+    objslc = np.array([i % 2 for i in range(nslices)], np.int32)
+    det_objslc, ndetslc = detection_planes(ndetper, nslices)
+    return Setup(nx=n, ny=n, a0=(a_nm, a_nm, A_PEROVSKITE * 0.1), ht=ht, wl=wl, nvarslc=[nvar, nvar],
+                 gratings=gratings, propagators=[pro], proidx=[0, 0], objslc=objslc, det_objslc=det_objslc,
+                 ndetslc=ndetslc, detectors=dets, probe=probe, flags=flags, slice_thickness=dz_nm,
+                 sites=sites)
+
+
+def single_probe_256(**kw) -> Setup:
+    """256^2, 4x4 cells, 100 slices, no frozen phonons."""
+    return perovskite_setup(n=256, ncell=4, nslices=100, nvar=1, **kw)
+
+
+def stem_scan_512(**kw) -> Setup:
+    """512^2, 8x8 cells, 200 slices, 8 frozen-phonon variants per slice type."""
+    return perovskite_setup(n=512, ncell=8, nslices=200, nvar=8, **kw)
+
+
+def scan_positions(nscan: int = 64, ncells_scan: int = 2):
+    """Probe positions [nm] of an nscan x nscan raster over ncells_scan^2 cells
+    (row by row), starting on the A column at the origin."""
+    step = ncells_scan * A_PEROVSKITE * 0.1 / nscan
+    iy, ix = np.meshgrid(np.arange(nscan), np.arange(nscan), indexing="ij")
+    return (ix.ravel() * step).astype(np.float32), (iy.ravel() * step).astype(np.float32)
+
+
+def variant_sequences(sp: Setup, nseq: int, seed: int = 42) -> np.ndarray:
+    """nseq variant sequences from one generator (serial scan order)."""
+    nv = np.ascontiguousarray(sp.nvarslc, np.int32)
+    out = np.zeros((nseq, len(sp.objslc)), np.int32)
+    rc = lib().jms_variant_sequences(seed, len(nv), ptr(nv), len(sp.objslc), ptr(sp.objslc), nseq,
+                                     ptr(out))
+    assert rc == 0
+    return out

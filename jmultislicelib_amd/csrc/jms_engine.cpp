@@ -1,0 +1,763 @@
+// jms_engine.cpp -- implementation of the thin C-ABI in include/jms_hip.h:
+// device memory, twiddle tables, the batched slice-loop schedule (eager or
+// captured in a HIP graph) and the single-wave path used by CJMultiSlice.
+//
+// Schedule of one batch of W probe passes (all resident in HBM at once):
+//   col_pass INIT  (plane islc0 readout)          1 launch
+//   for j in [islc0, nobjslc):
+//     row_pass  (slice j grating, per-wave variant)  1 launch
+//     col_pass  (slice j propagator, plane j+1)       1 launch (EXIT for the last)
+//   combine partial readouts -> per-position results 1 launch
+// This is the reference loop JMultiSlice.cpp:4590-4667 with readout order
+// unchanged (plane j read before slice j is applied; exit plane last).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "jms_hip.h"
+#include "jms_kernels.h"
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // (re)allocate to hold at least nb bytes; contents undefined
+  hipError_t ensure(size_t nb) {
+    if (nb <= bytes && p) return hipSuccess;
+    release();
+    if (nb == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e == hipSuccess) bytes = nb;
+    else p = nullptr;
+    return e;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+struct Timer {
+  double ms = 0.0;
+  long long count = 0;
+};
+
+}  // namespace
+
+struct jms_hip_engine {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  // grid
+  int nx = 0, ny = 0;
+  long long n = 0;
+  float a0x = 0.f, a0y = 0.f, wl = 0.f;
+  DevBuf twx, twy;
+  // gratings
+  int npgx = 0, npgy = 0, nscslc = 0;
+  std::vector<int> nvarslc;
+  std::vector<long long> slcoffset;
+  DevBuf pgr;
+  // propagators
+  int npro = 0;
+  std::vector<int> proidx;
+  DevBuf pro;
+  // object slices
+  int nobjslc = 0;
+  std::vector<int> objslc;
+  // detection
+  int ndetslc = 0, ndet = 0, detflags = 0;
+  std::vector<int> det_objslc;
+  DevBuf detw;
+  // incident wave
+  DevBuf wav0, cppl;
+  bool have_cppl = false;
+  // staged scan
+  int s_npos = 0, s_npass = 0;
+  DevBuf s_ppar, s_weights, s_var, s_res;
+  // batch working buffers
+  int bcap_req = 0;
+  DevBuf waves, partial, cur_var, cur_ppar, cur_res;
+  // single-wave path
+  DevBuf wav, var1, res1, w1, dif1, wff1;
+  // graph cache: key = (nwave, islc0)
+  bool use_graph = false;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  // timing
+  bool timing = false;
+  Timer tm[3];
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  size_t ev_next = 0;
+
+  int fail(int code, const char* what, hipError_t e) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s: %s (%d)", what, e == hipSuccess ? "" : hipGetErrorString(e), (int)e);
+    err = buf;
+    fprintf(stderr, "jms_hip error: %s\n", buf);
+    return code;
+  }
+  int fail(int code, const char* what) {
+    err = what;
+    fprintf(stderr, "jms_hip error: %s\n", what);
+    return code;
+  }
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+  int ntiles_col() const { return jms::col_tiles(ny, nx); }
+};
+
+namespace {
+
+void make_twiddles(int n, std::vector<float>& tw) {
+  tw.resize(2 * (size_t)n);
+  for (int e = 0; e < n; ++e) {
+    const double a = -2.0 * 3.14159265358979323846264338327950288 * (double)e / (double)n;
+    tw[2 * e] = (float)std::cos(a);
+    tw[2 * e + 1] = (float)std::sin(a);
+  }
+}
+
+// events for the optional per-launch timing
+hipEvent_t take_event(jms_hip_engine* e) {
+  if (e->ev_next >= e->ev_pool.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    e->ev_pool.push_back(ev);
+  }
+  return e->ev_pool[e->ev_next++];
+}
+
+void flush_timing(jms_hip_engine* e) {
+  if (e->ev_pending.empty()) return;
+  (void)hipStreamSynchronize(e->st);
+  for (auto& pe : e->ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+      e->tm[pe.first].ms += ms;
+      e->tm[pe.first].count += 1;
+    }
+  }
+  e->ev_pending.clear();
+  e->ev_next = 0;
+}
+
+struct TimedLaunch {
+  jms_hip_engine* e;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool on;
+  TimedLaunch(jms_hip_engine* e_, int kind_, bool on_) : e(e_), kind(kind_), on(on_) {
+    if (on) {
+      a = take_event(e);
+      b = take_event(e);
+      if (a && b) (void)hipEventRecord(a, e->st);
+      else on = false;
+    }
+  }
+  ~TimedLaunch() {
+    if (on) {
+      (void)hipEventRecord(b, e->st);
+      e->ev_pending.push_back({kind, {a, b}});
+      if (e->ev_pending.size() > 4096) flush_timing(e);
+    }
+  }
+};
+
+// Launch sequence of one batch.  `ppar` (float4 per wave) and `var` (nobjslc
+// ints per wave) describe the waves; src/src_stride the INIT source.
+// Readout partial sums go to e->partial, combined into `res` (npos results)
+// when npass > 0.
+int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long long src_stride,
+                  const void* ppar, const int* var, int npos, int npass, const float* weights,
+                  float* res, int accumulate, const float* cwgt, float* dif, void* wff,
+                  bool timed) {
+  const int S = e->nobjslc;
+  const int ntc = e->ntiles_col();
+  const long long part_w = (long long)e->ndetslc * e->ndet * ntc;
+  jms::ColArgsHost ca;
+  ca.wave = e->waves.p;
+  ca.nx = e->nx;
+  ca.wave_stride = e->n;
+  ca.kscx = 1.f / e->a0x;
+  ca.kscy = 1.f / e->a0y;
+  ca.cppl = e->have_cppl ? e->cppl.as<float>() : nullptr;
+  ca.ndet = (e->detflags & JMS_HIP_DETECT_INTEGRATED) ? e->ndet : 0;
+  ca.detw = e->detw.as<float>();
+  ca.partial = e->partial.as<float>();
+  ca.part_wstride = part_w;
+  ca.part_dstride = ntc;
+  ca.dif = dif;
+  ca.wff = wff;
+  ca.cwgt = cwgt;
+  ca.chan_wstride = (long long)e->ndetslc * e->n;
+  ca.twy = e->twy.p;
+  hipError_t err;
+  auto plane_setup = [&](int plane) {
+    const int id = (plane >= 0 && plane <= S) ? e->det_objslc[plane] : -1;
+    ca.detect = id >= 0 ? 1 : 0;
+    ca.part_off = id >= 0 ? (long long)id * e->ndet * ntc : 0;
+    ca.chan_off = id >= 0 ? (long long)id * e->n : 0;
+  };
+  // INIT column pass: plane islc0
+  {
+    ca.src = src;
+    ca.src_stride = src_stride;
+    ca.ppar = ppar;
+    ca.pro = nullptr;
+    plane_setup(islc0);
+    TimedLaunch tl(e, 1, timed);
+    err = jms::launch_col_pass(e->ny, nwave, true, islc0 >= S, ca, e->st);
+    if (err != hipSuccess) return e->fail(130, "column pass (init)", err);
+  }
+  ca.src = nullptr;
+  ca.ppar = nullptr;
+  for (int j = islc0; j < S; ++j) {
+    const int js = e->objslc[j] % e->nscslc;
+    {
+      TimedLaunch tl(e, 0, timed);
+      err = jms::launch_row_pass(e->nx, e->ny, nwave, e->waves.as<float2>(), e->n,
+                                 e->pgr.as<float2>(), e->slcoffset[js], (long long)e->npgx * e->npgy,
+                                 var, S, j, e->twx.as<float2>(), e->st);
+      if (err != hipSuccess) return e->fail(131, "row pass", err);
+    }
+    ca.pro = e->pro.as<float2>() + (long long)e->proidx[js] * e->n;
+    plane_setup(j + 1);
+    {
+      TimedLaunch tl(e, 1, timed);
+      err = jms::launch_col_pass(e->ny, nwave, false, j == S - 1, ca, e->st);
+      if (err != hipSuccess) return e->fail(132, "column pass", err);
+    }
+  }
+  if (npass > 0 && res && ca.ndet > 0) {
+    TimedLaunch tl(e, 2, timed);
+    err = jms::launch_combine(e->partial.as<float>(), ntc, e->ndet, e->ndetslc, npos, npass, weights,
+                              res, accumulate, e->st);
+    if (err != hipSuccess) return e->fail(133, "combine", err);
+  }
+  return 0;
+}
+
+int check_ready(jms_hip_engine* e) {
+  if (!e) return 1;
+  if (e->n <= 0) return e->fail(2, "grid not set");
+  if (!e->pgr.p && e->nobjslc > 0) return e->fail(3, "phase gratings not set up");
+  if (!e->pro.p && e->nobjslc > 0) return e->fail(4, "propagators not set up");
+  if (e->det_objslc.size() != (size_t)e->nobjslc + 1) return e->fail(5, "detection not set up");
+  return 0;
+}
+
+int ensure_batch(jms_hip_engine* e, int nwave) {
+  hipError_t err = e->waves.ensure(sizeof(float) * 2 * (size_t)e->n * nwave);
+  if (err != hipSuccess) return e->fail(110, "allocate wave batch", err);
+  const size_t np = sizeof(float) * (size_t)nwave * std::max(1, e->ndetslc) *
+                    std::max(1, e->ndet) * (size_t)e->ntiles_col();
+  err = e->partial.ensure(np);
+  if (err != hipSuccess) return e->fail(111, "allocate readout partials", err);
+  return 0;
+}
+
+// default batch: about 2 GiB of resident waves
+int auto_batch(jms_hip_engine* e, int total_waves) {
+  if (e->bcap_req > 0) return std::min(total_waves, e->bcap_req);
+  const long long per = 8LL * e->n;
+  long long cap = (2LL << 30) / std::max(1LL, per);
+  cap = std::max(1LL, std::min(cap, 8192LL));
+  return (int)std::min<long long>(total_waves, cap);
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int jms_hip_create(jms_hip_engine** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  jms_hip_engine* e = new jms_hip_engine();
+  hipError_t err = hipGetDevice(&e->device);
+  if (err != hipSuccess) { delete e; return 101; }
+  err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  if (err != hipSuccess) { delete e; return 102; }
+  *out = e;
+  return 0;
+}
+
+void jms_hip_destroy(jms_hip_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  e->drop_graphs();
+  for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+  DevBuf* bufs[] = {&e->twx, &e->twy, &e->pgr, &e->pro, &e->detw, &e->wav0, &e->cppl,
+                    &e->s_ppar, &e->s_weights, &e->s_var, &e->s_res, &e->waves, &e->partial,
+                    &e->cur_var, &e->cur_ppar, &e->cur_res, &e->wav, &e->var1, &e->res1,
+                    &e->w1, &e->dif1, &e->wff1};
+  for (DevBuf* b : bufs) b->release();
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+}
+
+int jms_hip_grid_supported(int nx, int ny) {
+  return (jms::size_supported(nx) && jms::size_supported(ny)) ? 1 : 0;
+}
+
+const char* jms_hip_last_error(jms_hip_engine* e) { return e ? e->err.c_str() : "no engine"; }
+
+int jms_hip_set_grid(jms_hip_engine* e, int nx, int ny, float a0x, float a0y, float wl) {
+  if (!e) return 1;
+  if (!jms_hip_grid_supported(nx, ny)) return e->fail(2, "grid size has no compiled kernels");
+  if (!(a0x > 0.f) || !(a0y > 0.f)) return e->fail(3, "invalid super-cell size");
+  (void)hipSetDevice(e->device);
+  e->drop_graphs();
+  e->nx = nx; e->ny = ny; e->n = (long long)nx * ny;
+  e->a0x = a0x; e->a0y = a0y; e->wl = wl;
+  std::vector<float> tw;
+  make_twiddles(nx, tw);
+  hipError_t err = e->twx.ensure(tw.size() * sizeof(float));
+  if (err == hipSuccess) err = hipMemcpy(e->twx.p, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(101, "twiddles x", err);
+  make_twiddles(ny, tw);
+  err = e->twy.ensure(tw.size() * sizeof(float));
+  if (err == hipSuccess) err = hipMemcpy(e->twy.p, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "twiddles y", err);
+  return 0;
+}
+
+int jms_hip_setup_gratings(jms_hip_engine* e, int npgx, int npgy, int nslc, const int* nvarslc) {
+  if (!e) return 1;
+  if (nslc < 1 || !nvarslc) return e->fail(2, "no phase gratings");
+  if (npgx != e->nx || npgy != e->ny)
+    return e->fail(3, "phase-grating grid differs from the wave grid (sub-frame gratings are not supported)");
+  (void)hipSetDevice(e->device);
+  e->drop_graphs();
+  e->npgx = npgx; e->npgy = npgy; e->nscslc = nslc;
+  e->nvarslc.assign(nvarslc, nvarslc + nslc);
+  e->slcoffset.assign(nslc, 0);
+  long long off = 0;
+  for (int i = 0; i < nslc; ++i) {
+    if (e->nvarslc[i] < 1) return e->fail(4, "structure slice without variants");
+    e->slcoffset[i] = off;
+    off += (long long)e->nvarslc[i] * npgx * npgy;
+  }
+  hipError_t err = e->pgr.ensure(sizeof(float) * 2 * (size_t)off);
+  if (err != hipSuccess) return e->fail(104, "allocate phase gratings", err);
+  err = hipMemset(e->pgr.p, 0, sizeof(float) * 2 * (size_t)off);
+  if (err != hipSuccess) return e->fail(105, "clear phase gratings", err);
+  return 0;
+}
+
+int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* pgr) {
+  if (!e) return 1;
+  if (islc < 0 || islc >= e->nscslc || !pgr || nvar < 1) return e->fail(2, "invalid grating upload");
+  const int nl = std::min(nvar, e->nvarslc[islc]);
+  const size_t nb = sizeof(float) * 2 * (size_t)nl * e->npgx * e->npgy;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipMemcpy(e->pgr.as<float2>() + e->slcoffset[islc], pgr, nb, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "upload phase gratings", err);
+  return 0;
+}
+
+int jms_hip_setup_propagators(jms_hip_engine* e, int npro, int nslc, const int* proidx) {
+  if (!e) return 1;
+  if (npro < 1 || !proidx || nslc < 1) return e->fail(2, "no propagators");
+  for (int i = 0; i < nslc; ++i)
+    if (proidx[i] < 0 || proidx[i] >= npro) return e->fail(3, "propagator index out of range");
+  (void)hipSetDevice(e->device);
+  e->drop_graphs();
+  e->npro = npro;
+  e->proidx.assign(proidx, proidx + nslc);
+  hipError_t err = e->pro.ensure(sizeof(float) * 2 * (size_t)npro * e->n);
+  if (err != hipSuccess) return e->fail(102, "allocate propagators", err);
+  return 0;
+}
+
+int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro) {
+  if (!e) return 1;
+  if (ipro < 0 || ipro >= e->npro || !pro) return e->fail(2, "invalid propagator upload");
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipMemcpy(e->pro.as<float2>() + (long long)ipro * e->n, pro,
+                             sizeof(float) * 2 * (size_t)e->n, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "upload propagator", err);
+  return 0;
+}
+
+int jms_hip_setup_objects(jms_hip_engine* e, int nobjslc, const int* objslc) {
+  if (!e) return 1;
+  if (nobjslc < 0 || (nobjslc > 0 && !objslc)) return e->fail(2, "invalid object slices");
+  for (int i = 0; i < nobjslc; ++i)
+    if (objslc[i] < 0) return e->fail(3, "negative structure slice index");
+  e->drop_graphs();
+  e->nobjslc = nobjslc;
+  e->objslc.assign(objslc, objslc + nobjslc);
+  return 0;
+}
+
+int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objslc, int ndet, int flags) {
+  if (!e) return 1;
+  if (!det_objslc || ndetslc < 1 || ndet < 0) return e->fail(2, "invalid detection setup");
+  if (ndet > jms::kMaxDetPerSweep) return e->fail(3, "too many integrating detectors (max 8)");
+  if (flags & ~(JMS_HIP_DETECT_INTEGRATED | JMS_HIP_DETECT_DIFFRACTION | JMS_HIP_DETECT_WAVEFOURIER))
+    return e->fail(4, "unsupported detection channel (real-space channels are not implemented)");
+  for (int i = 0; i <= e->nobjslc; ++i)
+    if (det_objslc[i] >= ndetslc) return e->fail(5, "detection index out of range");
+  (void)hipSetDevice(e->device);
+  e->drop_graphs();
+  e->ndetslc = ndetslc;
+  e->ndet = ndet;
+  e->detflags = flags;
+  e->det_objslc.assign(det_objslc, det_objslc + e->nobjslc + 1);
+  if (ndet > 0) {
+    hipError_t err = e->detw.ensure(sizeof(float) * (size_t)ndet * e->n);
+    if (err == hipSuccess) err = hipMemset(e->detw.p, 0, sizeof(float) * (size_t)ndet * e->n);
+    if (err != hipSuccess) return e->fail(103, "allocate detector functions", err);
+  }
+  return 0;
+}
+
+int jms_hip_upload_detector(jms_hip_engine* e, int idet, const float* det, int msklen, const int* msk) {
+  if (!e) return 1;
+  if (idet < 0 || idet >= e->ndet || !det) return e->fail(2, "invalid detector upload");
+  (void)hipSetDevice(e->device);
+  float* dst = e->detw.as<float>() + (long long)idet * e->n;
+  const size_t nb = sizeof(float) * (size_t)e->n;
+  hipError_t err;
+  if (msk && msklen > 0) {
+    for (int i = 0; i < msklen; ++i)
+      if (msk[i] < 0 || msk[i] >= e->n) return e->fail(3, "detector mask index out of range");
+    DevBuf tdet, tmsk;
+    err = tdet.ensure(nb);
+    if (err == hipSuccess) err = tmsk.ensure(sizeof(int) * (size_t)msklen);
+    if (err == hipSuccess) err = hipMemcpy(tdet.p, det, nb, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(tmsk.p, msk, sizeof(int) * (size_t)msklen, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemsetAsync(dst, 0, nb, e->st);
+    if (err == hipSuccess) err = jms::launch_scatter_mask(dst, tdet.as<float>(), tmsk.as<int>(), msklen, e->st);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+    tdet.release();
+    tmsk.release();
+  } else {
+    err = hipMemcpy(dst, det, nb, hipMemcpyHostToDevice);
+  }
+  if (err != hipSuccess) return e->fail(104, "upload detector", err);
+  return 0;
+}
+
+int jms_hip_set_probe(jms_hip_engine* e, const float* wav0) {
+  if (!e) return 1;
+  if (!wav0 || e->n <= 0) return e->fail(2, "invalid probe");
+  (void)hipSetDevice(e->device);
+  const size_t nb = sizeof(float) * 2 * (size_t)e->n;
+  hipError_t err = e->wav0.ensure(nb);
+  if (err == hipSuccess) err = hipMemcpy(e->wav0.p, wav0, nb, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "upload probe", err);
+  return 0;
+}
+
+int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl) {
+  if (!e) return 1;
+  (void)hipSetDevice(e->device);
+  e->drop_graphs();
+  if (!cppl) {
+    e->have_cppl = false;
+    return 0;
+  }
+  const size_t nb = sizeof(float) * (size_t)e->n;
+  hipError_t err = e->cppl.ensure(nb);
+  if (err == hipSuccess) err = hipMemcpy(e->cppl.p, cppl, nb, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "upload phase plate", err);
+  e->have_cppl = true;
+  return 0;
+}
+
+int jms_hip_set_batch(jms_hip_engine* e, int nwave_max) {
+  if (!e) return 1;
+  e->bcap_req = std::max(0, nwave_max);
+  e->drop_graphs();
+  return 0;
+}
+
+int jms_hip_set_graph(jms_hip_engine* e, int use_graph) {
+  if (!e) return 1;
+  e->use_graph = use_graph != 0;
+  return 0;
+}
+
+int jms_hip_set_timing(jms_hip_engine* e, int enable) {
+  if (!e) return 1;
+  flush_timing(e);
+  e->timing = enable != 0;
+  for (auto& t : e->tm) { t.ms = 0.0; t.count = 0; }
+  return 0;
+}
+
+int jms_hip_get_timing(jms_hip_engine* e, int kind, double* ms, long long* count) {
+  if (!e || kind < 0 || kind > 2) return 1;
+  flush_timing(e);
+  if (ms) *ms = e->tm[kind].ms;
+  if (count) *count = e->tm[kind].count;
+  return 0;
+}
+
+int jms_hip_stage_scan(jms_hip_engine* e, int npos, const float* dx, const float* dy, const float* dz,
+                       int npass, const float* weights, const int* var) {
+  if (!e) return 1;
+  if (npos < 1 || npass < 1 || !dx || !dy || !weights) return e->fail(2, "invalid scan");
+  if (e->nobjslc > 0 && !var) return e->fail(3, "missing variant sequences");
+  int rc = check_ready(e);
+  if (rc) return rc;
+  // variant range check
+  const long long nv = (long long)npos * npass * e->nobjslc;
+  for (long long i = 0; i < nv; ++i) {
+    const int j = (int)(i % e->nobjslc);
+    const int js = e->objslc[j] % e->nscslc;
+    if (var[i] < 0 || var[i] >= e->nvarslc[js]) return e->fail(4, "variant index out of range");
+  }
+  (void)hipSetDevice(e->device);
+  std::vector<float> pp(4 * (size_t)npos * npass);
+  for (int p = 0; p < npos; ++p)
+    for (int r = 0; r < npass; ++r) {
+      float* q = &pp[4 * ((size_t)p * npass + r)];
+      q[0] = dx[p];
+      q[1] = dy[p];
+      q[2] = (dz ? dz[p] : 0.f) * e->wl * 0.5f;  // fz = dz * lambda / 2 (JMultiSlice.cpp:1973)
+      q[3] = 0.f;
+    }
+  hipError_t err = e->s_ppar.ensure(pp.size() * sizeof(float));
+  if (err == hipSuccess) err = hipMemcpy(e->s_ppar.p, pp.data(), pp.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = e->s_weights.ensure(sizeof(float) * npass);
+  if (err == hipSuccess) err = hipMemcpy(e->s_weights.p, weights, sizeof(float) * npass, hipMemcpyHostToDevice);
+  if (err == hipSuccess && nv > 0) err = e->s_var.ensure(sizeof(int) * (size_t)nv);
+  if (err == hipSuccess && nv > 0) err = hipMemcpy(e->s_var.p, var, sizeof(int) * (size_t)nv, hipMemcpyHostToDevice);
+  const size_t nres = sizeof(float) * (size_t)npos * e->ndetslc * std::max(1, e->ndet);
+  if (err == hipSuccess) err = e->s_res.ensure(nres);
+  if (err == hipSuccess) err = hipMemset(e->s_res.p, 0, nres);
+  if (err != hipSuccess) return e->fail(102, "stage scan", err);
+  e->s_npos = npos;
+  e->s_npass = npass;
+  return 0;
+}
+
+int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
+  if (!e) return 1;
+  if (pos0 < 0 || npos < 1 || pos0 + npos > e->s_npos) return e->fail(2, "position range outside the staged scan");
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->wav0.p) return e->fail(6, "probe not set");
+  islc0 = std::max(0, std::min(islc0, e->nobjslc));
+  (void)hipSetDevice(e->device);
+  const int npass = e->s_npass;
+  const int total = npos * npass;
+  int bw = auto_batch(e, total);
+  int bpos = std::max(1, bw / npass);  // whole positions per batch
+  bw = bpos * npass;
+  rc = ensure_batch(e, bw);
+  if (rc) return rc;
+  const int S = e->nobjslc;
+  const int per = e->ndetslc * std::max(1, e->ndet);
+  for (int p = pos0; p < pos0 + npos; p += bpos) {
+    const int np = std::min(bpos, pos0 + npos - p);
+    const int nw = np * npass;
+    const float4* ppar = e->s_ppar.as<float4>() + (long long)p * npass;
+    const int* var = e->s_var.as<int>() + (long long)p * npass * S;
+    float* res = e->s_res.as<float>() + (long long)p * per;
+    if (e->use_graph && !e->timing) {
+      // the graph reads fixed "current batch" buffers, refreshed before replay
+      hipError_t err = e->cur_ppar.ensure(sizeof(float4) * (size_t)bw);
+      if (err == hipSuccess && S > 0) err = e->cur_var.ensure(sizeof(int) * (size_t)bw * S);
+      if (err == hipSuccess) err = e->cur_res.ensure(sizeof(float) * (size_t)bpos * per);
+      if (err == hipSuccess) err = hipMemcpyAsync(e->cur_ppar.p, ppar, sizeof(float4) * nw, hipMemcpyDeviceToDevice, e->st);
+      if (err == hipSuccess && S > 0)
+        err = hipMemcpyAsync(e->cur_var.p, var, sizeof(int) * (size_t)nw * S, hipMemcpyDeviceToDevice, e->st);
+      if (err != hipSuccess) return e->fail(120, "stage batch", err);
+      auto key = std::make_pair(nw, islc0);
+      auto it = e->graphs.find(key);
+      if (it == e->graphs.end()) {
+        hipGraph_t g = nullptr;
+        err = hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal);
+        if (err != hipSuccess) return e->fail(121, "begin capture", err);
+        rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, e->cur_ppar.p, e->cur_var.as<int>(), np,
+                           npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, nullptr,
+                           nullptr, nullptr, false);
+        hipError_t err2 = hipStreamEndCapture(e->st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (err2 != hipSuccess) return e->fail(122, "end capture", err2);
+        hipGraphExec_t ge = nullptr;
+        err = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) return e->fail(123, "instantiate graph", err);
+        it = e->graphs.emplace(key, ge).first;
+      }
+      err = hipGraphLaunch(it->second, e->st);
+      if (err == hipSuccess)
+        err = hipMemcpyAsync(res, e->cur_res.p, sizeof(float) * (size_t)np * per, hipMemcpyDeviceToDevice, e->st);
+      if (err != hipSuccess) return e->fail(124, "graph launch", err);
+    } else {
+      rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, ppar, var, np, npass, e->s_weights.as<float>(), res,
+                         0, nullptr, nullptr, nullptr, e->timing);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+int jms_hip_sync(jms_hip_engine* e) {
+  if (!e) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(101, "synchronize", err);
+  flush_timing(e);
+  return 0;
+}
+
+int jms_hip_get_scan_results(jms_hip_engine* e, float* out) {
+  if (!e || !out) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  const size_t nb = sizeof(float) * (size_t)e->s_npos * e->ndetslc * e->ndet;
+  if (err == hipSuccess && nb > 0) err = hipMemcpy(out, e->s_res.p, nb, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(102, "read scan results", err);
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// single-wave path
+// --------------------------------------------------------------------------
+int jms_hip_wave_offset(jms_hip_engine* e, float dx, float dy, float dz, int ibtx, int ibty, int use_cppl) {
+  if (!e) return 1;
+  if (!e->wav0.p) return e->fail(2, "probe not set");
+  (void)hipSetDevice(e->device);
+  hipError_t err = e->wav.ensure(sizeof(float) * 2 * (size_t)e->n);
+  if (err != hipSuccess) return e->fail(103, "allocate wave", err);
+  int ish0 = ibtx % e->nx; if (ish0 < 0) ish0 += e->nx;  // JMultiSlice.cpp:1990-1991
+  int ish1 = ibty % e->ny; if (ish1 < 0) ish1 += e->ny;
+  err = jms::launch_offset_wave(e->wav.as<float2>(), e->wav0.as<float2>(), e->nx, e->ny, 1.f / e->a0x,
+                                1.f / e->a0y, dx, dy, dz * e->wl * 0.5f,
+                                (use_cppl && e->have_cppl) ? e->cppl.as<float>() : nullptr, ish0, ish1, e->st);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(106, "offset incoming wave", err);
+  return 0;
+}
+
+int jms_hip_wave_set(jms_hip_engine* e, const float* wav) {
+  if (!e || !wav) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = e->wav.ensure(sizeof(float) * 2 * (size_t)e->n);
+  if (err == hipSuccess) err = hipMemcpy(e->wav.p, wav, sizeof(float) * 2 * (size_t)e->n, hipMemcpyHostToDevice);
+  if (err != hipSuccess) return e->fail(102, "set wave", err);
+  return 0;
+}
+
+int jms_hip_wave_get(jms_hip_engine* e, float* dst) {
+  if (!e || !dst) return 1;
+  if (!e->wav.p) return e->fail(2, "no wave");
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipMemcpy(dst, e->wav.p, sizeof(float) * 2 * (size_t)e->n, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(102, "get wave", err);
+  return 0;
+}
+
+int jms_hip_multislice(jms_hip_engine* e, int islc0, const int* var, float weight, int accumulate) {
+  if (!e) return 1;
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->wav.p) return e->fail(6, "no incoming wave (call jms_hip_wave_offset or jms_hip_wave_set)");
+  const int S = e->nobjslc;
+  if (S > 0 && !var) return e->fail(7, "missing variant sequence");
+  for (int j = 0; j < S; ++j) {
+    const int js = e->objslc[j] % e->nscslc;
+    if (var[j] < 0 || var[j] >= e->nvarslc[js]) return e->fail(8, "variant index out of range");
+  }
+  islc0 = std::max(0, std::min(islc0, S));
+  (void)hipSetDevice(e->device);
+  rc = ensure_batch(e, 1);
+  if (rc) return rc;
+  const int per = e->ndetslc * std::max(1, e->ndet);
+  const size_t nchan = (size_t)e->ndetslc * e->n;
+  hipError_t err = e->res1.ensure(sizeof(float) * per);
+  if (err == hipSuccess) err = e->w1.ensure(sizeof(float));
+  if (err == hipSuccess && S > 0) err = e->var1.ensure(sizeof(int) * S);
+  const bool wdif = (e->detflags & JMS_HIP_DETECT_DIFFRACTION) != 0;
+  const bool wwff = (e->detflags & JMS_HIP_DETECT_WAVEFOURIER) != 0;
+  if (err == hipSuccess && wdif) err = e->dif1.ensure(sizeof(float) * nchan);
+  if (err == hipSuccess && wwff) err = e->wff1.ensure(sizeof(float) * 2 * nchan);
+  if (err != hipSuccess) return e->fail(110, "allocate single-wave buffers", err);
+  if (!accumulate) {  // ClearDetMem_d (JMultiSlice.cpp:4054-4108)
+    err = hipMemsetAsync(e->res1.p, 0, sizeof(float) * per, e->st);
+    if (err == hipSuccess && wdif) err = hipMemsetAsync(e->dif1.p, 0, sizeof(float) * nchan, e->st);
+    if (err == hipSuccess && wwff) err = hipMemsetAsync(e->wff1.p, 0, sizeof(float) * 2 * nchan, e->st);
+    if (err != hipSuccess) return e->fail(111, "clear detector channels", err);
+  }
+  err = hipMemcpyAsync(e->w1.p, &weight, sizeof(float), hipMemcpyHostToDevice, e->st);
+  if (err == hipSuccess && S > 0) err = hipMemcpyAsync(e->var1.p, var, sizeof(int) * S, hipMemcpyHostToDevice, e->st);
+  if (err != hipSuccess) return e->fail(112, "stage single wave", err);
+  rc = enqueue_batch(e, 1, islc0, e->wav.p, e->n, nullptr, e->var1.as<int>(), 1, 1, e->w1.as<float>(),
+                     e->res1.as<float>(), 1, e->w1.as<float>(), wdif ? e->dif1.as<float>() : nullptr,
+                     wwff ? e->wff1.p : nullptr, e->timing);
+  if (rc) return rc;
+  // the run leaves psi(k) of the exit plane in the batch slot; keep it as the wave
+  err = hipMemcpyAsync(e->wav.p, e->waves.p, sizeof(float) * 2 * (size_t)e->n, hipMemcpyDeviceToDevice, e->st);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(113, "multislice run", err);
+  return 0;
+}
+
+int jms_hip_get_result(jms_hip_engine* e, int which, float* dst) {
+  if (!e || !dst) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  const size_t nchan = (size_t)e->ndetslc * e->n;
+  if (err != hipSuccess) return e->fail(101, "synchronize", err);
+  if (which == JMS_HIP_DETECT_INTEGRATED) {
+    if (!e->res1.p) return e->fail(2, "no result");
+    err = hipMemcpy(dst, e->res1.p, sizeof(float) * (size_t)e->ndetslc * e->ndet, hipMemcpyDeviceToHost);
+  } else if (which == JMS_HIP_DETECT_DIFFRACTION) {
+    if (!e->dif1.p) return e->fail(3, "diffraction channel not active");
+    err = hipMemcpy(dst, e->dif1.p, sizeof(float) * nchan, hipMemcpyDeviceToHost);
+  } else if (which == JMS_HIP_DETECT_WAVEFOURIER) {
+    if (!e->wff1.p) return e->fail(4, "wave channel not active");
+    err = hipMemcpy(dst, e->wff1.p, sizeof(float) * 2 * nchan, hipMemcpyDeviceToHost);
+  } else {
+    return e->fail(5, "unsupported result channel");
+  }
+  if (err != hipSuccess) return e->fail(102, "read result", err);
+  return 0;
+}
+
+int jms_hip_fft2d(int nx, int ny, int nbatch, float* data, int dir) {
+  if (!data || nbatch < 1 || dir == 0) return 1;
+  if (!jms_hip_grid_supported(nx, ny)) return 2;
+  std::vector<float> tx, ty;
+  make_twiddles(nx, tx);
+  make_twiddles(ny, ty);
+  const size_t nb = sizeof(float) * 2 * (size_t)nx * ny * nbatch;
+  DevBuf d, dtx, dty;
+  hipError_t err = d.ensure(nb);
+  if (err == hipSuccess) err = dtx.ensure(tx.size() * sizeof(float));
+  if (err == hipSuccess) err = dty.ensure(ty.size() * sizeof(float));
+  if (err == hipSuccess) err = hipMemcpy(d.p, data, nb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(dtx.p, tx.data(), tx.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(dty.p, ty.data(), ty.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = jms::launch_fft2d(nx, ny, nbatch, d.as<float2>(), dir, dtx.as<float2>(), dty.as<float2>(), nullptr);
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err == hipSuccess) err = hipMemcpy(data, d.p, nb, hipMemcpyDeviceToHost);
+  d.release(); dtx.release(); dty.release();
+  if (err != hipSuccess) {
+    fprintf(stderr, "jms_hip_fft2d: %s\n", hipGetErrorString(err));
+    return 101;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// jms_kernels.h -- host-side launchers of the multislice kernels (jms_kernels.hip)
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace jms {
+
+// Arguments of one column pass (see k_col_pass in jms_kernels.hip).
+struct ColArgsHost {
+  void* wave = nullptr;          // batch of waves (float2), wave_stride items apart
+  int nx = 0;
+  long long wave_stride = 0;
+  const void* pro = nullptr;     // propagator of this slice (float2, nx*ny)
+  const void* src = nullptr;     // INIT source waves (float2)
+  long long src_stride = 0;      // 0: one shared source; nx*ny: one per batch entry
+  const void* ppar = nullptr;    // INIT: float4 (dx, dy, fz, 0) per batch entry, or nullptr
+  float kscx = 0.f, kscy = 0.f;  // 1/a0x, 1/a0y [1/nm]
+  const float* cppl = nullptr;   // constant phase plate [rad] or nullptr
+  int detect = 0;                // readout at this plane
+  int ndet = 0;
+  const float* detw = nullptr;   // [ndet][nx*ny] effective detector weights
+  float* partial = nullptr;      // partial readout sums
+  long long part_wstride = 0;    // per batch entry
+  long long part_dstride = 0;    // per detector (= number of column tiles)
+  long long part_off = 0;        // offset of this detection plane
+  float* dif = nullptr;          // diffraction channel |psi|^2 (or nullptr)
+  void* wff = nullptr;           // Fourier-space wave channel (float2, or nullptr)
+  const float* cwgt = nullptr;   // per batch entry weight of the channels
+  long long chan_wstride = 0;    // per batch entry stride of the channels (items)
+  long long chan_off = 0;        // offset of this detection plane in the channels
+  const void* twy = nullptr;     // twiddles W_ny^e (float2, ny)
+};
+
+constexpr int kMaxDetPerSweep = 8;
+
+bool size_supported(int n);
+int row_tiles(int nx, int ny);
+int col_tiles(int ny, int nx);
+
+hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
+                           const float2* pgr, long long slc_off, long long npg, const int* var_tab,
+                           int var_ld, int jslc, const float2* twx, hipStream_t st);
+hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColArgsHost& a,
+                           hipStream_t st);
+hipError_t launch_fft2d(int nx, int ny, int nbatch, float2* data, int dir, const float2* twx,
+                        const float2* twy, hipStream_t st);
+hipError_t launch_combine(const float* partial, int ntiles, int ndet, int ndetslc, int npos,
+                          int npass, const float* weights, float* out, int accumulate,
+                          hipStream_t st);
+hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, float kscx,
+                              float kscy, float dx, float dy, float fz, const float* cppl,
+                              int ish0, int ish1, hipStream_t st);
+hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st);
+hipError_t launch_scatter_mask(float* detw, const float* det, const int* msk, int msklen,
+                               hipStream_t st);
+
+}  // namespace jms

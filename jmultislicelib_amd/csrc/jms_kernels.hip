@@ -1,0 +1,207 @@
+// jms_kernels.hip -- size dispatch of the line-length templated kernels and
+// the small non-templated kernels (readout combine, incident-wave offset,
+// accumulation, detector weight scatter).
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "jms_fft_device.hpp"
+#include "jms_kernels.h"
+#include "jms_sizes.h"
+#include "jms_cfg.hpp"
+
+namespace jms {
+
+template <int N>
+hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
+                             const float2* pgr, long long slc_off, long long npg,
+                             const int* var_tab, int var_ld, int jslc, const float2* twx,
+                             hipStream_t st);
+template <int N>
+hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost& h,
+                             hipStream_t st);
+template <int N>
+hipError_t launch_row_fft_n(int ny, int nbatch, float2* data, long long stride, int dir,
+                            const float2* twx, hipStream_t st);
+template <int N>
+hipError_t launch_col_fft_n(int nx, int nbatch, float2* data, long long stride, int dir,
+                            const float2* twy, hipStream_t st);
+
+// frequency index of position i on an n-point grid in FFT order
+// (JMultiSlice.cpp:1699-1712)
+__device__ __forceinline__ int fidx(int i, int n) {
+  const int h = (n - (n & 1)) >> 1;
+  return ((i + h) % n) - h;
+}
+
+// ---------------------------------------------------------------------------
+// combine partial detector sums of a batch into per-position results:
+//   out[(p0+p), idet, d] (+)= sum_r weight[r] * sum_tile partial[(p*npass+r), idet, d, tile]
+// ---------------------------------------------------------------------------
+__global__ void k_combine(const float* __restrict__ partial, int ntiles, int ndet, int ndetslc,
+                          int npos, int npass, const float* __restrict__ weights,
+                          float* __restrict__ out, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = ndetslc * ndet;
+  if (i >= npos * per) return;
+  const int p = i / per, r0 = i % per;  // r0 = idet*ndet + d
+  double acc = 0.0;
+  for (int r = 0; r < npass; ++r) {
+    const float* src = partial + ((long long)(p * npass + r) * per + r0) * ntiles;
+    double s = 0.0;
+    for (int k = 0; k < ntiles; ++k) s += (double)src[k];
+    acc += (double)weights[r] * s;
+  }
+  out[i] = accumulate ? (float)((double)out[i] + acc) : (float)acc;
+}
+
+// psi_out = psi0 * exp(-i [2 pi (kx dx + ky dy + k^2 fz) + cppl]) in Fourier space
+// (OffsetIncomingWave GPU branch, JMultiSlice.cpp:2036-2077; ArrayOps.cu:390-422),
+// with an optional cyclic beam-tilt shift of psi0 by (ish0, ish1) pixels
+// (CShift2dKernel, ArrayOps.cu:340-352).
+__global__ void k_offset_wave(float2* __restrict__ out, const float2* __restrict__ in, int nx,
+                              int ny, float kscx, float kscy, float dx, float dy, float fz,
+                              const float* __restrict__ cppl, int ish0, int ish1) {
+  const long long n = (long long)nx * ny;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int i = (int)(idx % nx), j = (int)(idx / nx);
+  const float kx = (float)fidx(i, nx) * kscx, ky = (float)fidx(j, ny) * kscy;
+  const float ksqr = kx * kx + ky * ky;
+  float chi = -6.28318531f * (dx * kx + dy * ky + fz * ksqr);
+  if (cppl) chi -= cppl[idx];
+  float sn, cs;
+  sincosf(chi, &sn, &cs);
+  int i0 = i - ish0, j0 = j - ish1;
+  i0 %= nx; if (i0 < 0) i0 += nx;
+  j0 %= ny; if (j0 < 0) j0 += ny;
+  out[idx] = cmul(in[(long long)i0 + (long long)j0 * nx], make_float2(cs, sn));
+}
+
+// dst += w * src (complex), n items
+__global__ void k_axpy_c(float2* __restrict__ dst, const float2* __restrict__ src, long long n, float w) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 s = src[i];
+  float2 d = dst[i];
+  d.x += w * s.x;
+  d.y += w * s.y;
+  dst[i] = d;
+}
+
+// effective detector weights: detw[d][idx] += det[idx] for every mask entry
+// (mask given), or detw[d][:] = det[:] (no mask); see jms_hip_upload_detector.
+__global__ void k_scatter_mask(float* __restrict__ detw, const float* __restrict__ det,
+                               const int* __restrict__ msk, int msklen) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= msklen) return;
+  const int idx = msk[i];
+  atomicAdd(&detw[idx], det[idx]);
+}
+
+
+// ---------------------------------------------------------------------------
+// dispatch over the compiled line lengths
+// ---------------------------------------------------------------------------
+bool size_supported(int n) {
+  switch (n) {
+#define JMS_CASE(N) case N: return true;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return false;
+  }
+}
+
+int row_tiles(int nx, int ny) {
+  switch (nx) {
+#define JMS_CASE(N) case N: return (ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return 0;
+  }
+}
+
+int col_tiles(int ny, int nx) {
+  switch (ny) {
+#define JMS_CASE(N) case N: return (nx + Cfg<N>::C - 1) / Cfg<N>::C;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return 0;
+  }
+}
+
+hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
+                           const float2* pgr, long long slc_off, long long npg, const int* var_tab,
+                           int var_ld, int jslc, const float2* twx, hipStream_t st) {
+  switch (nx) {
+#define JMS_CASE(N)                                                                         \
+  case N:                                                                                   \
+    return launch_row_pass_n<N>(ny, nwave, wave, wave_stride, pgr, slc_off, npg, var_tab, \
+                                var_ld, jslc, twx, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColArgsHost& a,
+                           hipStream_t st) {
+  switch (ny) {
+#define JMS_CASE(N) case N: return launch_col_pass_n<N>(nwave, init, exit_, a, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fft2d(int nx, int ny, int nbatch, float2* data, int dir, const float2* twx,
+                        const float2* twy, hipStream_t st) {
+  const long long stride = (long long)nx * ny;
+  hipError_t e = hipErrorInvalidValue;
+  switch (nx) {
+#define JMS_CASE(N) case N: e = launch_row_fft_n<N>(ny, nbatch, data, stride, dir, twx, st); break;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  switch (ny) {
+#define JMS_CASE(N) case N: return launch_col_fft_n<N>(nx, nbatch, data, stride, dir, twy, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_combine(const float* partial, int ntiles, int ndet, int ndetslc, int npos,
+                          int npass, const float* weights, float* out, int accumulate,
+                          hipStream_t st) {
+  const int total = npos * ndetslc * ndet;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_combine, dim3((total + 255) / 256), dim3(256), 0, st, partial, ntiles, ndet,
+                     ndetslc, npos, npass, weights, out, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, float kscx,
+                              float kscy, float dx, float dy, float fz, const float* cppl,
+                              int ish0, int ish1, hipStream_t st) {
+  const long long n = (long long)nx * ny;
+  hipLaunchKernelGGL(k_offset_wave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, in,
+                     nx, ny, kscx, kscy, dx, dy, fz, cppl, ish0, ish1);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_mask(float* detw, const float* det, const int* msk, int msklen,
+                               hipStream_t st) {
+  if (msklen <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_mask, dim3((msklen + 255) / 256), dim3(256), 0, st, detw, det, msk,
+                     msklen);
+  return hipGetLastError();
+}
+
+}  // namespace jms

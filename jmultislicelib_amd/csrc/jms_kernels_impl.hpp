@@ -1,0 +1,486 @@
+// jms_kernels_impl.hpp -- HIP kernels of the batched multislice slice loop (gfx950).
+//
+// The wave functions of a batch of probe passes are stored one after the
+// other in HBM, each as ny rows of nx complex64 values (index x + y*nx, x
+// fastest; Fourier-space arrays in FFT order), exactly the layout of the
+// reference's device wave m_d_wav (JMultiSlice.cpp:1772-1797).
+//
+// One slice of the reference loop (JMultiSlice.cpp:4590-4651) is
+//     [readout] -> IFT -> * grating -> FT -> * propagator
+// and is executed here as two fused passes over the wave:
+//   row pass    : per row y:     IFFT_x, * t(x,y), FFT_x           (k_row_pass)
+//   column pass : per column kx: FFT_y, * P(kx,ky), [readout], IFFT_y (k_col_pass)
+// The column pass of slice j also performs the readout of plane j+1 and the
+// first half (IFFT_y) of slice j+1's inverse transform, so each slice costs two
+// read+write sweeps of the wave.  The first column pass of a run (INIT) builds
+// the incident wave from the probe and the per-position phase plate
+// (OffsetIncomingWave, JMultiSlice.cpp:1961-2084; ArrayOps.cu:390-422); the
+// last one (EXIT) stops after the exit-plane readout and leaves psi(k).
+//
+// The kernels are templates on the line length; jms_kernels_size.hip
+// instantiates them once per supported length (one translation unit each).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "jms_fft_device.hpp"
+#include "jms_kernels.h"
+#include "jms_cfg.hpp"
+
+namespace jms {
+
+constexpr int kMaxDet = 8;  // detectors handled per readout sweep
+
+struct SyncBlock {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+
+// frequency index of position i on an n-point grid in FFT order
+// (JMultiSlice.cpp:1699-1712)
+__device__ __forceinline__ int fidx(int i, int n) {
+  const int h = (n - (n & 1)) >> 1;
+  return ((i + h) % n) - h;
+}
+
+// ---------------------------------------------------------------------------
+// row pass: wave[w] rows <- FFT_x( t[jslc, var] * IFFT_x(rows) )
+// grid: (ceil(ny/LPB), nwave); block: T*LPB
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_pass(
+    float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
+    long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
+    const float2* __restrict__ twg) {
+  using C_ = Cfg<N>;
+  using P = typename C_::P;
+  constexpr int T = C_::T, LPB = C_::LPB;
+  using L = LineLayout<N, C_::PADSH>;
+  constexpr int NREG = regs_needed<N, T, P>();
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(LPB)];
+  const int tid = threadIdx.x;
+  const int t = tid % T, line = tid / T;
+  for (int e = tid; e < N; e += T * LPB) s_tw[e] = twg[e];
+  const long long w = blockIdx.y;
+  const int row = blockIdx.x * LPB + line;
+  const bool valid = row < ny;
+  float2* wrow = wave + w * wave_stride + (long long)row * N;
+  const int var = var_tab[w * var_ld + jslc];
+  const float2* grow = pgr + slc_off + (long long)var * npg + (long long)row * N;
+  using M0 = IOMap<N, T, P, false>;
+  using M1 = IOMap<N, T, P, true>;
+  float2 v[NREG];
+  if (valid) {
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF) {
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M0::R0 + q] = wrow[b + q * M0::NBF];
+        });
+      }
+    });
+  }
+  __syncthreads();
+  LineFFT<N, T, +1, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  if (valid) {
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF) {
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M1::R0 + q] = cmul(v[i * M1::R0 + q], grow[b + q * M1::NBF]);
+        });
+      }
+    });
+  }
+  LineFFT<N, T, -1, P, true, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  if (valid) {
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF) {
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          wrow[b + q * M0::NBF] = v[i * M0::R0 + q];
+        });
+      }
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block reduction of kMaxDet per-thread partial sums; thread 0 writes
+// partial[d * dstride] for d < ndet.  red: LDS scratch of >= 64*kMaxDet floats.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_reduce_det(float (&acc)[kMaxDet], int ndet, float* red,
+                                                 float* out, long long dstride) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int d = 0; d < kMaxDet; ++d) {
+    float x = acc[d];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    acc[d] = x;
+  }
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < kMaxDet; ++d) red[wv * kMaxDet + d] = acc[d];
+  }
+  __syncthreads();
+  if (threadIdx.x < ndet && threadIdx.x < kMaxDet) {
+    const int d = threadIdx.x;
+    float s = 0.f;
+    for (int i = 0; i < nw; ++i) s += red[i * kMaxDet + d];
+    out[d * dstride] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column pass.
+//   INIT : psi <- src (shared probe, or one wave per batch entry) times the
+//          phase plate exp(-i[2 pi (kx dx + ky dy + k^2 fz) + cppl]) when ppar
+//          is given (MulPhasePlateC01Kernel, ArrayOps.cu:406-422)
+//   else : psi <- P * FFT_y(wave)
+//   detect: partial[(w, idet, d, tile)] = sum_{column tile} |psi|^2 * detw[d]
+//   EXIT : store psi(k); else store IFFT_y(psi)
+// grid: (ceil(nx/C), nwave); block: TC*C
+// ---------------------------------------------------------------------------
+struct ColArgs {
+  float2* wave;
+  int nx;
+  long long wave_stride;
+  const float2* pro;        // propagator of this slice (not INIT)
+  const float2* src;        // INIT source
+  long long src_stride;     // 0: shared probe; n: per-wave input
+  const float4* ppar;       // INIT: per-wave (dx, dy, fz, 0) or nullptr
+  float kscx, kscy;         // 1/a0x, 1/a0y [1/nm]
+  const float* cppl;        // constant phase plate (or nullptr)
+  int detect;               // 1: readout at this plane
+  int ndet;
+  const float* detw;        // [ndet][n] effective detector weights
+  float* partial;           // partial sums
+  long long part_wstride;   // per-wave stride in partial
+  long long part_dstride;   // per-detector stride (= ntiles)
+  long long part_off;       // plane offset (idetslc * ndet * ntiles)
+  float* dif;               // diffraction channel (or nullptr)
+  float2* wff;              // Fourier-space wave channel (or nullptr)
+  const float* cwgt;        // per-wave channel weight
+  long long chan_wstride;   // per-wave stride of the channels (items)
+  long long chan_off;       // plane offset in the channels (items)
+  const float2* twg;
+};
+
+template <int N, bool INIT, bool EXIT>
+__global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_pass(ColArgs a) {
+  using C_ = Cfg<N>;
+  using P = typename C_::P;
+  constexpr int T = C_::TC, C = C_::C;
+  using L = TileLayout<N, C, C_::PADSH>;
+  constexpr int NREG = regs_needed<N, T, P>();
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(1)];
+  const int tid = threadIdx.x;
+  const int c = tid % C, t = tid / C;
+  for (int e = tid; e < N; e += T * C) s_tw[e] = a.twg[e];
+  const long long w = blockIdx.y;
+  const int nx = a.nx;
+  const int x = blockIdx.x * C + c;
+  const bool valid = x < nx;
+  float2* wcol = a.wave + w * a.wave_stride + x;
+  using M0 = IOMap<N, T, P, false>;
+  using M1 = IOMap<N, T, P, true>;
+  float2 v[NREG];
+  if constexpr (INIT) {
+    if (valid) {
+      const float2* s = a.src + w * a.src_stride + x;
+      float dx = 0.f, dy = 0.f, fz = 0.f;
+      const bool pp = a.ppar != nullptr;
+      if (pp) {
+        const float4 q4 = a.ppar[w];
+        dx = q4.x; dy = q4.y; fz = q4.z;
+      }
+      const float kx = (float)fidx(x, nx) * a.kscx;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const int e = b + q * M1::NBF;
+            float2 val = s[(long long)e * nx];
+            if (pp) {
+              const float ky = (float)fidx(e, N) * a.kscy;
+              const float ksqr = kx * kx + ky * ky;
+              float chi = -6.28318531f * (dx * kx + dy * ky + fz * ksqr);
+              if (a.cppl) chi -= a.cppl[(long long)e * nx + x];
+              float sn, cs;
+              sincosf(chi, &sn, &cs);
+              val = cmul(val, make_float2(cs, sn));
+            }
+            v[i * M1::R0 + q] = val;
+          });
+        }
+      });
+    }
+    __syncthreads();
+  } else {
+    if (valid) {
+      static_for<0, M0::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M0::FULL || b < M0::NBF) {
+          static_for<0, M0::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            v[i * M0::R0 + q] = wcol[(long long)(b + q * M0::NBF) * nx];
+          });
+        }
+      });
+    }
+    __syncthreads();
+    LineFFT<N, T, -1, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+    if (valid) {
+      const float2* pcol = a.pro + x;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            v[i * M1::R0 + q] = cmul(v[i * M1::R0 + q], pcol[(long long)(b + q * M1::NBF) * nx]);
+          });
+        }
+      });
+    }
+  }
+  if (a.detect) {
+    float acc[kMaxDet];
+#pragma unroll
+    for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
+    if (valid) {
+      const long long n = (long long)N * nx;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const long long idx = (long long)(b + q * M1::NBF) * nx + x;
+            const float2 z = v[i * M1::R0 + q];
+            const float p = z.x * z.x + z.y * z.y;
+#pragma unroll
+            for (int d = 0; d < kMaxDet; ++d)
+              if (d < a.ndet) acc[d] += p * a.detw[d * n + idx];
+          });
+        }
+      });
+    }
+    block_reduce_det(acc, a.ndet, (float*)s_buf,
+                     a.partial + w * a.part_wstride + a.part_off + blockIdx.x, a.part_dstride);
+    __syncthreads();
+    if (valid && (a.dif || a.wff)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
+      const float wg = a.cwgt[w];
+      const long long cb = w * a.chan_wstride + a.chan_off;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
+            const float2 z = v[i * M1::R0 + q];
+            if (a.dif) a.dif[idx] += wg * (z.x * z.x + z.y * z.y);
+            if (a.wff) {
+              float2 o = a.wff[idx];
+              o.x += z.x * wg;
+              o.y += z.y * wg;
+              a.wff[idx] = o;
+            }
+          });
+        }
+      });
+    }
+  }
+  if constexpr (EXIT) {
+    if (valid) {
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            wcol[(long long)(b + q * M1::NBF) * nx] = v[i * M1::R0 + q];
+          });
+        }
+      });
+    }
+  } else {
+    LineFFT<N, T, +1, P, true, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+    if (valid) {
+      static_for<0, M0::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M0::FULL || b < M0::NBF) {
+          static_for<0, M0::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            wcol[(long long)(b + q * M0::NBF) * nx] = v[i * M0::R0 + q];
+          });
+        }
+      });
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// standalone line transforms (2-D FFT = row transform + column transform)
+// ---------------------------------------------------------------------------
+template <int N, int DIR>
+__global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_fft(float2* __restrict__ data,
+                                                                      int ny, long long stride,
+                                                                      const float2* __restrict__ twg) {
+  using C_ = Cfg<N>;
+  using P = typename C_::P;
+  constexpr int T = C_::T, LPB = C_::LPB;
+  using L = LineLayout<N, C_::PADSH>;
+  constexpr int NREG = regs_needed<N, T, P>();
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(LPB)];
+  const int tid = threadIdx.x, t = tid % T, line = tid / T;
+  for (int e = tid; e < N; e += T * LPB) s_tw[e] = twg[e];
+  const int row = blockIdx.x * LPB + line;
+  const bool valid = row < ny;
+  float2* d = data + blockIdx.y * stride + (long long)row * N;
+  using M0 = IOMap<N, T, P, false>;
+  using M1 = IOMap<N, T, P, true>;
+  float2 v[NREG];
+  if (valid) {
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF)
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M0::R0 + q] = d[b + q * M0::NBF];
+        });
+    });
+  }
+  __syncthreads();
+  LineFFT<N, T, DIR, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  if (valid) {
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF)
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          d[b + q * M1::NBF] = v[i * M1::R0 + q];
+        });
+    });
+  }
+}
+
+template <int N, int DIR>
+__global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_fft(float2* __restrict__ data,
+                                                                     int nx, long long stride,
+                                                                     const float2* __restrict__ twg) {
+  using C_ = Cfg<N>;
+  using P = typename C_::P;
+  constexpr int T = C_::TC, C = C_::C;
+  using L = TileLayout<N, C, C_::PADSH>;
+  constexpr int NREG = regs_needed<N, T, P>();
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(1)];
+  const int tid = threadIdx.x, c = tid % C, t = tid / C;
+  for (int e = tid; e < N; e += T * C) s_tw[e] = twg[e];
+  const int x = blockIdx.x * C + c;
+  const bool valid = x < nx;
+  float2* d = data + blockIdx.y * stride + x;
+  using M0 = IOMap<N, T, P, false>;
+  using M1 = IOMap<N, T, P, true>;
+  float2 v[NREG];
+  if (valid) {
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF)
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M0::R0 + q] = d[(long long)(b + q * M0::NBF) * nx];
+        });
+    });
+  }
+  __syncthreads();
+  LineFFT<N, T, DIR, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+  if (valid) {
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF)
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          d[(long long)(b + q * M1::NBF) * nx] = v[i * M1::R0 + q];
+        });
+    });
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// per-size launchers (instantiated in jms_kernels_size.hip)
+// ---------------------------------------------------------------------------
+template <int N>
+hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
+                             const float2* pgr, long long slc_off, long long npg,
+                             const int* var_tab, int var_ld, int jslc, const float2* twx,
+                             hipStream_t st) {
+  dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nwave), b(Cfg<N>::T * Cfg<N>::LPB);
+  hipLaunchKernelGGL(k_row_pass<N>, g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
+                     var_tab, var_ld, jslc, twx);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost& h,
+                             hipStream_t st) {
+  ColArgs a;
+  a.wave = (float2*)h.wave; a.nx = h.nx; a.wave_stride = h.wave_stride;
+  a.pro = (const float2*)h.pro; a.src = (const float2*)h.src; a.src_stride = h.src_stride;
+  a.ppar = (const float4*)h.ppar; a.kscx = h.kscx; a.kscy = h.kscy; a.cppl = h.cppl;
+  a.detect = h.detect; a.ndet = h.ndet; a.detw = h.detw; a.partial = h.partial;
+  a.part_wstride = h.part_wstride; a.part_dstride = h.part_dstride; a.part_off = h.part_off;
+  a.dif = h.dif; a.wff = (float2*)h.wff; a.cwgt = h.cwgt; a.chan_wstride = h.chan_wstride;
+  a.chan_off = h.chan_off;
+  a.twg = (const float2*)h.twy;
+  dim3 g((h.nx + Cfg<N>::C - 1) / Cfg<N>::C, nwave), b(Cfg<N>::TC * Cfg<N>::C);
+  if (init && exit_) hipLaunchKernelGGL((k_col_pass<N, true, true>), g, b, 0, st, a);
+  else if (init) hipLaunchKernelGGL((k_col_pass<N, true, false>), g, b, 0, st, a);
+  else if (exit_) hipLaunchKernelGGL((k_col_pass<N, false, true>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_col_pass<N, false, false>), g, b, 0, st, a);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_row_fft_n(int ny, int nbatch, float2* data, long long stride, int dir,
+                            const float2* twx, hipStream_t st) {
+  dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nbatch), b(Cfg<N>::T * Cfg<N>::LPB);
+  if (dir < 0) hipLaunchKernelGGL((k_row_fft<N, -1>), g, b, 0, st, data, ny, stride, twx);
+  else hipLaunchKernelGGL((k_row_fft<N, +1>), g, b, 0, st, data, ny, stride, twx);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_col_fft_n(int nx, int nbatch, float2* data, long long stride, int dir,
+                            const float2* twy, hipStream_t st) {
+  dim3 g((nx + Cfg<N>::C - 1) / Cfg<N>::C, nbatch), b(Cfg<N>::TC * Cfg<N>::C);
+  if (dir < 0) hipLaunchKernelGGL((k_col_fft<N, -1>), g, b, 0, st, data, nx, stride, twy);
+  else hipLaunchKernelGGL((k_col_fft<N, +1>), g, b, 0, st, data, nx, stride, twy);
+  return hipGetLastError();
+}
+
+template <int N> constexpr int row_lines_per_block() { return Cfg<N>::LPB; }
+template <int N> constexpr int col_lines_per_block() { return Cfg<N>::C; }
+
+}  // namespace jms

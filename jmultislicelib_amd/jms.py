@@ -1,0 +1,342 @@
+"""Python host-side mirror of the reference multislice interface.
+
+``JMultiSlice`` forwards to the flat C-ABI of libjmultislice_hip.so (the
+reference's exported function names, src/JMultiSliceLib.h:111-380) and keeps
+their argument meaning and 0/>0 error codes.  ``Engine`` wraps the thin
+engine ABI (include/jms_hip.h) used for batched scans and timing.
+All arrays are numpy; complex arrays are complex64 of shape (ny, nx) in FFT
+order, the reference's memory layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CJProbeParams, lib, ptr
+
+_JMS_CODE_CPU = 1
+_JMS_CODE_GPU = 2
+_JMS_ACCMODE_NONE = 0
+_JMS_ACCMODE_INTEGRATE = 1
+_JMS_DETECT_INTEGRATED = 1
+_JMS_DETECT_DIFFRACTION = 4
+_JMS_DETECT_WAVEFOURIER = 16
+
+
+class JMSError(RuntimeError):
+    def __init__(self, fn: str, code: int):
+        super().__init__(f"{fn} returned error code {code}")
+        self.code = code
+
+
+def _chk(fn: str, rc: int) -> int:
+    if rc != 0:
+        raise JMSError(fn, rc)
+    return rc
+
+
+def _c64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex64)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class JMultiSlice:
+    """Facade over the library's process-global CJMultiSlice (the flat C-ABI)."""
+
+    def __init__(self):
+        self.L = lib()
+
+    # -- physics / numerics ------------------------------------------------
+    def SetHighTension(self, ht: float) -> float:
+        return self.L.SetHighTension(ht)
+
+    def GetWaveLength(self) -> float:
+        return self.L.GetWaveLength()
+
+    def SetSupercellSize(self, a: float, b: float, c: float) -> None:
+        self.L.SetSupercellSizeABC(a, b, c)
+
+    def SetGridSize(self, nx: int, ny: int) -> None:
+        self.L.SetGridSize(nx, ny)
+
+    def GetGridSize(self):
+        a, b = C.c_int(), C.c_int()
+        self.L.GetGridSize(C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def SeedRngEx(self, seed: int) -> None:
+        self.L.SeedRngEx(seed)
+
+    def GetDetetionSliceNum(self) -> int:
+        return self.L.GetDetetionSliceNum()
+
+    def GetDetNum(self) -> int:
+        return self.L.GetDetNum()
+
+    # -- calculators ---------------------------------------------------------
+    def CalculatePropagator(self, fthick: float, otx: float, oty: float, ntype: int = 0) -> np.ndarray:
+        nx, ny = self.GetGridSize()
+        pro = np.zeros((ny, nx), np.complex64)
+        _chk("CalculatePropagator", self.L.CalculatePropagator(fthick, otx, oty, ptr(pro), ntype))
+        return pro
+
+    def CalculateRingDetector(self, beta0, beta1, phi0=0.0, phi1=0.0, theta0x=0.0, theta0y=0.0,
+                              sdsprofile: str = "", kmom1: int = 0, mask: bool = True):
+        nx, ny = self.GetGridSize()
+        det = np.zeros((ny, nx), np.float32)
+        msk = np.zeros(nx * ny, np.int32) if mask else None
+        ml = C.c_int(0)
+        _chk("CalculateRingDetector",
+             self.L.CalculateRingDetector(beta0, beta1, phi0, phi1, theta0x, theta0y,
+                                          sdsprofile.encode(), ptr(det), C.byref(ml), ptr(msk), kmom1))
+        return det, (msk[: ml.value].copy() if mask else None)
+
+    def CalculateProbeWaveFourier(self, alpha_mrad: float, alpha_rs: float = 0.03, alpha_x0=0.0,
+                                  alpha_y0=0.0, alpha_asym=0.0, alpha_adir=0.0, btx=0.0,
+                                  bty=0.0) -> np.ndarray:
+        nx, ny = self.GetGridSize()
+        prm = CJProbeParams()
+        prm.m_alpha = alpha_mrad
+        prm.m_alpha_rs = alpha_rs
+        prm.m_alpha_x0 = alpha_x0
+        prm.m_alpha_y0 = alpha_y0
+        prm.m_alpha_asym = alpha_asym
+        prm.m_alpha_adir = alpha_adir
+        prm.m_btx = btx
+        prm.m_bty = bty
+        prm.m_abrr_num = 0
+        wav = np.zeros((ny, nx), np.complex64)
+        _chk("CalculateProbeWaveFourier", self.L.CalculateProbeWaveFourier(C.byref(prm), ptr(wav)))
+        return wav
+
+    def GetUnscrambleHash(self) -> np.ndarray:
+        nx, ny = self.GetGridSize()
+        h = np.zeros(nx * ny, np.uint32)
+        _chk("GetUnscrambleHash", self.L.GetUnscrambleHash(ptr(h)))
+        return h
+
+    # -- setup chain -----------------------------------------------------------
+    def PhaseGratingSetup(self, whichcode, nx, ny, nslc, nvarmax, nslcvar) -> int:
+        v = _i32(nslcvar)
+        return self.L.PhaseGratingSetup(whichcode, nx, ny, nslc, nvarmax, ptr(v))
+
+    def ObjectSliceSetup(self, objslc) -> int:
+        o = _i32(objslc)
+        return self.L.ObjectSliceSetup(len(o), ptr(o))
+
+    def PropagatorSetup(self, whichcode, npro, proidx) -> int:
+        p = _i32(proidx)
+        return self.L.PropagatorSetup(whichcode, npro, ptr(p))
+
+    def DetectorSetup(self, whichcode, ndetper, ndetint, imagedet, nthreads_CPU=1) -> int:
+        return self.L.DetectorSetup(whichcode, ndetper, ndetint, imagedet, nthreads_CPU)
+
+    def SetPhaseGratingData(self, whichcode, islc, pgr) -> int:
+        g = _c64(pgr)
+        nvar = g.shape[0] if g.ndim == 3 else 1
+        return self.L.SetSlicePhaseGratings(whichcode, islc, nvar, ptr(g))
+
+    def SetPropagatorData(self, whichcode, ipro, pro) -> int:
+        p = _c64(pro)
+        return self.L.SetPropagatorData(whichcode, ipro, ptr(p))
+
+    def SetDetectorData(self, whichcode, idet, det, msk=None) -> int:
+        d = _f32(det)
+        m = _i32(msk) if msk is not None else None
+        return self.L.SetDetectorData(whichcode, idet, ptr(d), 0 if m is None else len(m), ptr(m))
+
+    def InitCore(self, whichcode, ncpu=1) -> int:
+        return self.L.InitCore(whichcode, ncpu)
+
+    def SetIncidentWave(self, whichcode, wav, transpose=False) -> int:
+        w = _c64(wav)
+        return self.L.SetIncidentWave(whichcode, ptr(w), int(transpose))
+
+    def SetIncidentWavePhaseplate(self, whichcode, ppl, transpose=False) -> int:
+        p = _f32(ppl)
+        return self.L.SetIncidentWavePhaseplate(whichcode, ptr(p), int(transpose))
+
+    # -- multislice --------------------------------------------------------------
+    def OffsetIncomingWave(self, whichcode, dx, dy, dz, iThread=0, ibtx=0, ibty=0, bcppl=True) -> int:
+        return self.L.OffsetIncomingWave(whichcode, dx, dy, dz, iThread, ibtx, ibty, int(bcppl))
+
+    def SetIncomingWaveGPU(self, wav, transpose=False) -> int:
+        w = _c64(wav)
+        return self.L.SetIncomingWaveGPU(ptr(w), int(transpose))
+
+    def GPUMultislice(self, islc0: int, accmode: int, weight: float = 1.0) -> int:
+        return self.L.GPUMultislice(islc0, accmode, weight)
+
+    def GetResult(self, whichcode, whichresult) -> np.ndarray:
+        nx, ny = self.GetGridSize()
+        nds = self.GetDetetionSliceNum()
+        if whichresult == _JMS_DETECT_INTEGRATED:
+            out = np.zeros((nds, self.GetDetNum()), np.float32)
+        elif whichresult == _JMS_DETECT_DIFFRACTION:
+            out = np.zeros((nds, ny, nx), np.float32)
+        elif whichresult == _JMS_DETECT_WAVEFOURIER:
+            out = np.zeros((nds, ny, nx), np.complex64)
+        else:
+            raise ValueError("unsupported result channel")
+        _chk("GetResult", self.L.GetResult(whichcode, whichresult, ptr(out), 0))
+        return out
+
+    def GetRandomVariantSequence(self, nobjslc: int) -> np.ndarray:
+        v = np.zeros(nobjslc, np.int32)
+        _chk("GetRandomVariantSequence", self.L.GetRandomVariantSequence(ptr(v)))
+        return v
+
+    def GPUMultisliceScan(self, dx, dy, dz, weights, var=None) -> np.ndarray:
+        dx, dy = _f32(dx), _f32(dy)
+        dz = _f32(dz) if dz is not None else None
+        w = _f32(weights)
+        npos, npass = len(dx), len(w)
+        out = np.zeros((npos, self.GetDetetionSliceNum(), self.GetDetNum()), np.float32)
+        if var is None:
+            rc = self.L.GPUMultisliceScan(npos, ptr(dx), ptr(dy), ptr(dz), npass, ptr(w), ptr(out))
+        else:
+            v = _i32(var)
+            rc = self.L.GPUMultisliceScanVar(npos, ptr(dx), ptr(dy), ptr(dz), npass, ptr(w), ptr(v),
+                                             ptr(out))
+        _chk("GPUMultisliceScan", rc)
+        return out
+
+    def Cleanup(self) -> int:
+        return self.L.Cleanup()
+
+
+class Engine:
+    """Batched multislice engine (include/jms_hip.h) on the current GPU."""
+
+    def __init__(self):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.jms_hip_create(C.byref(h))
+        if rc != 0:
+            raise JMSError("jms_hip_create", rc)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.jms_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, name, *args):
+        rc = getattr(self.L, name)(self.h, *args)
+        if rc != 0:
+            msg = self.L.jms_hip_last_error(self.h)
+            raise JMSError(f"{name} ({msg.decode() if msg else ''})", rc)
+
+    def setup(self, sp) -> "Engine":
+        """Upload a specimen/optics setup (jmultislicelib_amd.configs.Setup)."""
+        self._c("jms_hip_set_grid", sp.nx, sp.ny, sp.a0[0], sp.a0[1], sp.wl)
+        nv = _i32(sp.nvarslc)
+        self._c("jms_hip_setup_gratings", sp.nx, sp.ny, len(nv), ptr(nv))
+        for i, g in enumerate(sp.gratings):
+            g = _c64(g)
+            self._c("jms_hip_upload_gratings", i, g.shape[0], ptr(g))
+        pi = _i32(sp.proidx)
+        self._c("jms_hip_setup_propagators", len(sp.propagators), len(pi), ptr(pi))
+        for i, p in enumerate(sp.propagators):
+            p = _c64(p)
+            self._c("jms_hip_upload_propagator", i, ptr(p))
+        o = _i32(sp.objslc)
+        self._c("jms_hip_setup_objects", len(o), ptr(o))
+        d = _i32(sp.det_objslc)
+        self._c("jms_hip_setup_detection", sp.ndetslc, ptr(d), len(sp.detectors), sp.flags)
+        for i, (det, msk) in enumerate(sp.detectors):
+            det = _f32(det)
+            m = _i32(msk) if msk is not None else None
+            self._c("jms_hip_upload_detector", i, ptr(det), 0 if m is None else len(m), ptr(m))
+        w0 = _c64(sp.probe)
+        self._c("jms_hip_set_probe", ptr(w0))
+        self.sp = sp
+        return self
+
+    def stage_scan(self, dx, dy, dz, weights, var):
+        dx, dy = _f32(dx), _f32(dy)
+        dz = _f32(dz) if dz is not None else None
+        w = _f32(weights)
+        v = _i32(var)
+        self.npos = len(dx)
+        self._c("jms_hip_stage_scan", len(dx), ptr(dx), ptr(dy), ptr(dz), len(w), ptr(w), ptr(v))
+
+    def run_staged(self, pos0: int, npos: int, islc0: int = 0):
+        self._c("jms_hip_run_staged", pos0, npos, islc0)
+
+    def sync(self):
+        self._c("jms_hip_sync")
+
+    def scan_results(self) -> np.ndarray:
+        out = np.zeros((self.npos, self.sp.ndetslc, len(self.sp.detectors)), np.float32)
+        self._c("jms_hip_get_scan_results", ptr(out))
+        return out
+
+    def set_batch(self, nwave: int):
+        self._c("jms_hip_set_batch", nwave)
+
+    def set_graph(self, on: bool):
+        self._c("jms_hip_set_graph", int(on))
+
+    def set_timing(self, on: bool):
+        self._c("jms_hip_set_timing", int(on))
+
+    def timing(self, kind: int):
+        ms, cnt = C.c_double(), C.c_longlong()
+        self._c("jms_hip_get_timing", kind, C.byref(ms), C.byref(cnt))
+        return ms.value, cnt.value
+
+    def wave_offset(self, dx, dy, dz=0.0, ibtx=0, ibty=0, use_cppl=True):
+        self._c("jms_hip_wave_offset", dx, dy, dz, ibtx, ibty, int(use_cppl))
+
+    def wave_set(self, wav):
+        w = _c64(wav)
+        self._c("jms_hip_wave_set", ptr(w))
+
+    def wave_get(self) -> np.ndarray:
+        out = np.zeros((self.sp.ny, self.sp.nx), np.complex64)
+        self._c("jms_hip_wave_get", ptr(out))
+        return out
+
+    def multislice(self, var, weight=1.0, accumulate=False, islc0=0):
+        v = _i32(var) if var is not None and len(var) else np.zeros(1, np.int32)
+        self._c("jms_hip_multislice", islc0, ptr(v), weight, int(accumulate))
+
+    def result(self, which=_JMS_DETECT_INTEGRATED) -> np.ndarray:
+        sp = self.sp
+        if which == _JMS_DETECT_INTEGRATED:
+            out = np.zeros((sp.ndetslc, len(sp.detectors)), np.float32)
+        elif which == _JMS_DETECT_DIFFRACTION:
+            out = np.zeros((sp.ndetslc, sp.ny, sp.nx), np.float32)
+        else:
+            out = np.zeros((sp.ndetslc, sp.ny, sp.nx), np.complex64)
+        self._c("jms_hip_get_result", which, ptr(out))
+        return out
+
+
+def fft2d(a: np.ndarray, direction: int) -> np.ndarray:
+    """2-D FFT on the GPU of (..., ny, nx) complex64 data; direction < 0
+    forward exp(-i), > 0 backward exp(+i), unscaled."""
+    d = _c64(a).copy()
+    ny, nx = d.shape[-2:]
+    nb = int(np.prod(d.shape[:-2])) if d.ndim > 2 else 1
+    _chk("jms_hip_fft2d", lib().jms_hip_fft2d(nx, ny, nb, ptr(d), direction))
+    return d
+
+
+def grid_supported(nx: int, ny: int) -> bool:
+    return bool(lib().jms_hip_grid_supported(nx, ny))

@@ -1,0 +1,40 @@
+"""Shared helpers of the test suite: small setups and error measures."""
+import numpy as np
+
+from jmultislicelib_amd import configs
+from jmultislicelib_amd.configs import Setup
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def random_setup(nx, ny, nslices=8, nvar=(3, 2), ndetper=0, seed=0, flags=1, ht=200.0,
+                 ptype=1, det_ranges=((0, 10), (10, 25), (30, 120))) -> Setup:
+    """Generic setup on an nx x ny grid: random weak phase gratings, library
+    propagators, ring detectors and probe (used for non-square and
+    mixed-radix grids)."""
+    from jmultislicelib_amd import JMultiSlice
+    rng = np.random.default_rng(seed)
+    ax, ay = 0.012 * nx, 0.011 * ny  # ~0.12/0.11 A sampling, in nm
+    J = JMultiSlice()
+    J.SetHighTension(ht)
+    J.SetGridSize(nx, ny)
+    J.SetSupercellSize(ax, ay, 0.2)
+    wl = J.GetWaveLength()
+    gratings = []
+    for nv in nvar:
+        ph = rng.normal(0, 0.4, (nv, ny, nx))
+        gratings.append(np.exp(1j * ph).astype(np.complex64))
+    pros = [J.CalculatePropagator(0.19, 0.0, 0.0, ptype), J.CalculatePropagator(0.21, 0.0, 0.0, ptype)]
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in det_ranges]
+    probe = J.CalculateProbeWaveFourier(22.0, 0.03)
+    objslc = rng.integers(0, len(nvar), nslices).astype(np.int32)
+    det_objslc, ndetslc = configs.detection_planes(ndetper, nslices)
+    return Setup(nx=nx, ny=ny, a0=(ax, ay, 0.2), ht=ht, wl=wl, nvarslc=list(nvar), gratings=gratings,
+                 propagators=pros, proidx=[0, 1][: len(nvar)] + [0] * max(0, len(nvar) - 2),
+                 objslc=objslc, det_objslc=det_objslc, ndetslc=ndetslc, detectors=dets, probe=probe,
+                 flags=flags, slice_thickness=0.2)
