@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: STEM probe positions per second at 512^2 x 200 slices.
+
+Workload (stem_scan_512): 512x512 complex64 grid over 8x8 perovskite cells
+(31.24 A), 200 slices of two alternating types with 8 frozen-phonon variants
+each, 64x64 probe positions over 2x2 cells, 8 passes per position (weight
+1/8, variant sequences from one generator seeded 42), three annular
+detectors (0-11, 11-22, 70-200 mrad).  One step = 512 positions (4096 probe
+passes); the default 8 timed steps cover the full 4096-position scan.
+All inputs (gratings, propagator, detectors, probe, positions, variant
+sequences) are resident in HBM before the timed region.
+
+Prints ONE JSON line.  Kernel durations come from HIP events recorded around
+every launch on the engine's stream inside the timed region; the roofline
+object reports the dominant kernel's algorithmic bytes (16 * n per wave: one
+read and one write of the complex64 wave) per average launch duration against
+MI355X's 8 TB/s HBM peak.  cpu_baseline times the CPU oracle (this project's
+restatement of the reference CPUMultislice, float32, one std::thread per
+core) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "STEM probe-positions/s at 512^2 x 200 slices; surrogate vs plain at <=1% error"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sp, dx, dy, var, npass, cores):
+    """Oracle float32 scan on `cores` threads: 2*cores probe passes (one
+    position each), scaled to positions/s of npass passes."""
+    from oracle import oracle as O
+    nsample = 2 * cores
+    t0 = time.perf_counter()
+    O.stem_scan(sp, dx[:nsample], dy[:nsample], [1.0], var[:nsample], prec=32, nthreads=cores)
+    dt = time.perf_counter() - t0
+    passes_per_s = nsample / dt
+    return {"value": passes_per_s / npass, "unit": "positions/s", "cores": cores, "kind": "port",
+            "sample": f"{nsample} probe passes (one per position, 512^2, 200 slices, 3 detectors) "
+                      f"in {dt:.1f} s on {cores} threads; positions/s = passes/s / {npass}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--positions-per-step", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=2048, help="probe passes resident per batch")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay batches from a HIP graph (no per-kernel timing)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(positions shard independently across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from jmultislicelib_amd import Engine, configs
+
+    npass = 8
+    t0 = time.perf_counter()
+    sp = configs.stem_scan_512()
+    dx_all, dy_all = configs.scan_positions(64, 2)
+    npos_total = len(dx_all)
+    pps = args.positions_per_step
+    nsteps_needed = args.warmup + args.steps
+    # position p of step s: the scan is walked cyclically
+    order = np.arange(nsteps_needed * pps) % npos_total
+    dx, dy = dx_all[order], dy_all[order]
+    var = configs.variant_sequences(sp, len(order) * npass, seed=42)
+    log(f"setup {time.perf_counter() - t0:.1f} s: {sp.nx}x{sp.ny}, {len(sp.objslc)} slices, "
+        f"{len(order)} positions x {npass} passes")
+
+    eng = Engine().setup(sp)
+    eng.set_batch(args.batch)
+    eng.set_graph(args.graph)
+    weights = np.full(npass, 1.0 / npass, np.float32)
+    eng.stage_scan(dx, dy, None, weights, var)
+    eng.sync()
+
+    for s in range(args.warmup):
+        eng.run_staged(s * pps, pps)
+    eng.sync()
+    eng.set_timing(not args.graph)
+    t1 = time.perf_counter()
+    for s in range(args.warmup, nsteps_needed):
+        eng.run_staged(s * pps, pps)
+    eng.sync()
+    t2 = time.perf_counter()
+    wall = t2 - t1
+    res = eng.scan_results()
+    finite = bool(np.isfinite(res).all())
+    n = sp.nx * sp.ny
+    positions = args.steps * pps
+    value = positions / wall
+    roof = None
+    if not args.graph:
+        kinds = {0: "row_pass", 1: "col_pass", 2: "combine"}
+        tm = {k: eng.timing(k) for k in kinds}
+        for k, (ms, cnt) in tm.items():
+            log(f"kernel {kinds[k]}: {cnt} launches, {ms:.1f} ms total, {ms / max(cnt, 1):.3f} ms avg")
+        dom = max((0, 1), key=lambda k: tm[k][0])
+        ms, cnt = tm[dom]
+        nwave_avg = positions * npass * (len(sp.objslc) + (1 if dom == 1 else 0)) / max(cnt, 1)
+        bytes_per_launch = 16.0 * n * nwave_avg
+        achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": kinds[dom], "bytes_per_launch": bytes_per_launch,
+                "avg_launch_ms": round(ms / cnt, 4)}
+        total_kernel_ms = sum(v[0] for v in tm.values())
+        log(f"kernel time {total_kernel_ms / 1e3:.2f} s of {wall:.2f} s wall")
+    # whole-slice view: 32 n bytes per probe pass per slice
+    slice_bytes = 32.0 * n * positions * npass * len(sp.objslc)
+    log(f"scan: {positions} positions in {wall:.2f} s -> {value:.1f} positions/s; "
+        f"slice-loop HBM rate {slice_bytes / wall / 1e9:.0f} GB/s ({slice_bytes / wall / 1e9 / HBM_PEAK_GBS:.1%} of peak)")
+    log(f"results finite: {finite}; first position detectors {res[0, 0]}")
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cores = min(16, len(os.sched_getaffinity(0)))
+            cpu = cpu_baseline(sp, dx_all, dy_all, var, npass, cores)
+            log(f"cpu baseline: {cpu['value']:.3f} positions/s ({cpu['sample']})")
+        except Exception as ex:  # the baseline must not hide the GPU result
+            log(f"cpu baseline failed: {ex}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "positions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "stem_scan_512: 512x512 complex64, 200 slices (2 types x 8 frozen-phonon "
+                               "variants), 64x64 positions x 8 passes, 3 annular detectors",
+                   "grid": [sp.nx, sp.ny], "slices": int(len(sp.objslc)), "positions_per_step": pps,
+                   "passes": npass, "batch_waves": args.batch, "graph": bool(args.graph)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
