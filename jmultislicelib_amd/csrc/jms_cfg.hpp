@@ -29,6 +29,15 @@ struct Cfg {
   // number of 64-lane wavefronts (the readout reduction shuffles over 64 lanes)
   static constexpr int TC = (C >= 64) ? TC1 : (TC1 / (64 / C)) * (64 / C);
   static_assert(TC >= 1 && (TC * C) % 64 == 0, "column block must be whole wavefronts");
+  // wave-local column kernel: TW threads per column (largest power of two
+  // <= min(T, 64), so a column stays inside one wavefront), CW columns per
+  // block; used up to N = 768 (the staged tile must fit LDS)
+  static constexpr int pow2_floor(int v) { int p = 1; while (2 * p <= v) p *= 2; return p; }
+  static constexpr int TW = pow2_floor(T < 64 ? T : 64);
+  static constexpr int CW = 16;
+  static constexpr bool WL = (N <= 768) && ((TW * CW) % 64 == 0);
+  // columns per block of the column kernel in use (partial-sum tiles)
+  static constexpr int CC = WL ? CW : C;
   static constexpr int PADSH = 3;
 };
 

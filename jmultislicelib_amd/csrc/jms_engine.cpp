@@ -70,16 +70,16 @@ struct jms_hip_engine {
   // propagators
   int npro = 0;
   std::vector<int> proidx;
-  DevBuf pro;
+  DevBuf pro, pro_t;  // row-major and column-major ([x][y]) copies
   // object slices
   int nobjslc = 0;
   std::vector<int> objslc;
   // detection
   int ndetslc = 0, ndet = 0, detflags = 0;
   std::vector<int> det_objslc;
-  DevBuf detw;
+  DevBuf detw, detw_t;
   // incident wave
-  DevBuf wav0, cppl;
+  DevBuf wav0, cppl, cppl_t;
   bool have_cppl = false;
   // staged scan
   int s_npos = 0, s_npass = 0;
@@ -127,6 +127,15 @@ void make_twiddles(int n, std::vector<float>& tw) {
     tw[2 * e] = (float)std::cos(a);
     tw[2 * e + 1] = (float)std::sin(a);
   }
+}
+
+// column-major copy of a row-major ny x nx array (element (x, y) at x*ny + y)
+template <class T>
+std::vector<T> to_colmajor(const T* src, int nx, int ny) {
+  std::vector<T> out((size_t)nx * ny);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) out[(size_t)x * ny + y] = src[(size_t)y * nx + x];
+  return out;
 }
 
 // events for the optional per-launch timing
@@ -193,6 +202,8 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   ca.kscx = 1.f / e->a0x;
   ca.kscy = 1.f / e->a0y;
   ca.cppl = e->have_cppl ? e->cppl.as<float>() : nullptr;
+  ca.cppl_t = e->have_cppl ? e->cppl_t.as<float>() : nullptr;
+  ca.detw_t = e->detw_t.as<float>();
   ca.ndet = (e->detflags & JMS_HIP_DETECT_INTEGRATED) ? e->ndet : 0;
   ca.detw = e->detw.as<float>();
   ca.partial = e->partial.as<float>();
@@ -233,6 +244,7 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
       if (err != hipSuccess) return e->fail(131, "row pass", err);
     }
     ca.pro = e->pro.as<float2>() + (long long)e->proidx[js] * e->n;
+    ca.pro_t = e->pro_t.as<float2>() + (long long)e->proidx[js] * e->n;
     plane_setup(j + 1);
     {
       TimedLaunch tl(e, 1, timed);
@@ -300,7 +312,8 @@ void jms_hip_destroy(jms_hip_engine* e) {
   if (e->st) (void)hipStreamSynchronize(e->st);
   e->drop_graphs();
   for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
-  DevBuf* bufs[] = {&e->twx, &e->twy, &e->pgr, &e->pro, &e->detw, &e->wav0, &e->cppl,
+  DevBuf* bufs[] = {&e->twx, &e->twy, &e->pgr, &e->pro, &e->pro_t, &e->detw, &e->detw_t, &e->wav0,
+                    &e->cppl, &e->cppl_t,
                     &e->s_ppar, &e->s_weights, &e->s_var, &e->s_res, &e->waves, &e->partial,
                     &e->cur_var, &e->cur_ppar, &e->cur_res, &e->wav, &e->var1, &e->res1,
                     &e->w1, &e->dif1, &e->wff1};
@@ -379,6 +392,7 @@ int jms_hip_setup_propagators(jms_hip_engine* e, int npro, int nslc, const int* 
   e->npro = npro;
   e->proidx.assign(proidx, proidx + nslc);
   hipError_t err = e->pro.ensure(sizeof(float) * 2 * (size_t)npro * e->n);
+  if (err == hipSuccess) err = e->pro_t.ensure(sizeof(float) * 2 * (size_t)npro * e->n);
   if (err != hipSuccess) return e->fail(102, "allocate propagators", err);
   return 0;
 }
@@ -389,6 +403,11 @@ int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro) {
   (void)hipSetDevice(e->device);
   hipError_t err = hipMemcpy(e->pro.as<float2>() + (long long)ipro * e->n, pro,
                              sizeof(float) * 2 * (size_t)e->n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    std::vector<float2> t = to_colmajor((const float2*)pro, e->nx, e->ny);
+    err = hipMemcpy(e->pro_t.as<float2>() + (long long)ipro * e->n, t.data(), sizeof(float2) * t.size(),
+                    hipMemcpyHostToDevice);
+  }
   if (err != hipSuccess) return e->fail(102, "upload propagator", err);
   return 0;
 }
@@ -421,6 +440,8 @@ int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objsl
   if (ndet > 0) {
     hipError_t err = e->detw.ensure(sizeof(float) * (size_t)ndet * e->n);
     if (err == hipSuccess) err = hipMemset(e->detw.p, 0, sizeof(float) * (size_t)ndet * e->n);
+    if (err == hipSuccess) err = e->detw_t.ensure(sizeof(float) * (size_t)ndet * e->n);
+    if (err == hipSuccess) err = hipMemset(e->detw_t.p, 0, sizeof(float) * (size_t)ndet * e->n);
     if (err != hipSuccess) return e->fail(103, "allocate detector functions", err);
   }
   return 0;
@@ -430,25 +451,22 @@ int jms_hip_upload_detector(jms_hip_engine* e, int idet, const float* det, int m
   if (!e) return 1;
   if (idet < 0 || idet >= e->ndet || !det) return e->fail(2, "invalid detector upload");
   (void)hipSetDevice(e->device);
-  float* dst = e->detw.as<float>() + (long long)idet * e->n;
-  const size_t nb = sizeof(float) * (size_t)e->n;
-  hipError_t err;
+  // effective weights: det at every listed mask pixel (a pixel listed k times
+  // counts k times, as the masked sum does), or det everywhere without a mask
+  std::vector<float> wgt((size_t)e->n, 0.f);
   if (msk && msklen > 0) {
-    for (int i = 0; i < msklen; ++i)
+    for (int i = 0; i < msklen; ++i) {
       if (msk[i] < 0 || msk[i] >= e->n) return e->fail(3, "detector mask index out of range");
-    DevBuf tdet, tmsk;
-    err = tdet.ensure(nb);
-    if (err == hipSuccess) err = tmsk.ensure(sizeof(int) * (size_t)msklen);
-    if (err == hipSuccess) err = hipMemcpy(tdet.p, det, nb, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(tmsk.p, msk, sizeof(int) * (size_t)msklen, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemsetAsync(dst, 0, nb, e->st);
-    if (err == hipSuccess) err = jms::launch_scatter_mask(dst, tdet.as<float>(), tmsk.as<int>(), msklen, e->st);
-    if (err == hipSuccess) err = hipStreamSynchronize(e->st);
-    tdet.release();
-    tmsk.release();
+      wgt[msk[i]] += det[msk[i]];
+    }
   } else {
-    err = hipMemcpy(dst, det, nb, hipMemcpyHostToDevice);
+    std::memcpy(wgt.data(), det, sizeof(float) * (size_t)e->n);
   }
+  std::vector<float> wt = to_colmajor(wgt.data(), e->nx, e->ny);
+  const size_t nb = sizeof(float) * (size_t)e->n;
+  hipError_t err = hipMemcpy(e->detw.as<float>() + (long long)idet * e->n, wgt.data(), nb, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(e->detw_t.as<float>() + (long long)idet * e->n, wt.data(), nb, hipMemcpyHostToDevice);
   if (err != hipSuccess) return e->fail(104, "upload detector", err);
   return 0;
 }
@@ -473,8 +491,11 @@ int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl) {
     return 0;
   }
   const size_t nb = sizeof(float) * (size_t)e->n;
+  std::vector<float> t = to_colmajor(cppl, e->nx, e->ny);
   hipError_t err = e->cppl.ensure(nb);
   if (err == hipSuccess) err = hipMemcpy(e->cppl.p, cppl, nb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = e->cppl_t.ensure(nb);
+  if (err == hipSuccess) err = hipMemcpy(e->cppl_t.p, t.data(), nb, hipMemcpyHostToDevice);
   if (err != hipSuccess) return e->fail(102, "upload phase plate", err);
   e->have_cppl = true;
   return 0;

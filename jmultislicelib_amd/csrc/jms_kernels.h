@@ -28,6 +28,10 @@ struct ColArgsHost {
   long long chan_wstride = 0;    // per batch entry stride of the channels (items)
   long long chan_off = 0;        // offset of this detection plane in the channels
   const void* twy = nullptr;     // twiddles W_ny^e (float2, ny)
+  // column-major ([x][y]) copies for the wave-local column kernel
+  const void* pro_t = nullptr;   // propagator of this slice (float2)
+  const float* detw_t = nullptr; // [ndet][nx][ny] detector weights
+  const float* cppl_t = nullptr; // constant phase plate or nullptr
 };
 
 constexpr int kMaxDetPerSweep = 8;
@@ -50,7 +54,5 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
                               float kscy, float dx, float dy, float fz, const float* cppl,
                               int ish0, int ish1, hipStream_t st);
 hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st);
-hipError_t launch_scatter_mask(float* detw, const float* det, const int* msk, int msklen,
-                               hipStream_t st);
 
 }  // namespace jms

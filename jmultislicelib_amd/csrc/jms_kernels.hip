@@ -1,6 +1,6 @@
 // jms_kernels.hip -- size dispatch of the line-length templated kernels and
 // the small non-templated kernels (readout combine, incident-wave offset,
-// accumulation, detector weight scatter).
+// accumulation).
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "jms_fft_device.hpp"
@@ -87,17 +87,6 @@ __global__ void k_axpy_c(float2* __restrict__ dst, const float2* __restrict__ sr
   dst[i] = d;
 }
 
-// effective detector weights: detw[d][idx] += det[idx] for every mask entry
-// (mask given), or detw[d][:] = det[:] (no mask); see jms_hip_upload_detector.
-__global__ void k_scatter_mask(float* __restrict__ detw, const float* __restrict__ det,
-                               const int* __restrict__ msk, int msklen) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= msklen) return;
-  const int idx = msk[i];
-  atomicAdd(&detw[idx], det[idx]);
-}
-
-
 // ---------------------------------------------------------------------------
 // dispatch over the compiled line lengths
 // ---------------------------------------------------------------------------
@@ -121,7 +110,7 @@ int row_tiles(int nx, int ny) {
 
 int col_tiles(int ny, int nx) {
   switch (ny) {
-#define JMS_CASE(N) case N: return (nx + Cfg<N>::C - 1) / Cfg<N>::C;
+#define JMS_CASE(N) case N: return (nx + Cfg<N>::CC - 1) / Cfg<N>::CC;
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return 0;
@@ -193,14 +182,6 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
 hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_axpy_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n, w);
-  return hipGetLastError();
-}
-
-hipError_t launch_scatter_mask(float* detw, const float* det, const int* msk, int msklen,
-                               hipStream_t st) {
-  if (msklen <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter_mask, dim3((msklen + 255) / 256), dim3(256), 0, st, detw, det, msk,
-                     msklen);
   return hipGetLastError();
 }
 

@@ -34,6 +34,24 @@ struct SyncBlock {
   __device__ __forceinline__ void operator()() const { __syncthreads(); }
 };
 
+// Barrier for LDS regions private to one wavefront: the wave's earlier LDS
+// operations complete (they are processed in issue order) before later ones
+// issue; the memory clobber keeps the compiler from moving LDS accesses across.
+struct SyncWave {
+  __device__ __forceinline__ void operator()() const { __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+};
+
+// line-major LDS layout for column tiles: column c at c*STRIDE, element e at
+// e + (e >> PADSH); STRIDE = 2 (mod 32) so that the 16 columns of a staging
+// access (c fastest) fall on distinct bank pairs.
+template <int N, int PADSH>
+struct ColLayout {
+  static constexpr int base = N + (N >> PADSH) + 1;
+  static constexpr int STRIDE = base + ((34 - (base % 32)) % 32);
+  __device__ static __forceinline__ int idx(int line, int e) { return line * STRIDE + e + (e >> PADSH); }
+  static constexpr int words(int lines) { return lines * STRIDE; }
+};
+
 // frequency index of position i on an n-point grid in FFT order
 // (JMultiSlice.cpp:1699-1712)
 __device__ __forceinline__ int fidx(int i, int n) {
@@ -45,12 +63,11 @@ __device__ __forceinline__ int fidx(int i, int n) {
 // row pass: wave[w] rows <- FFT_x( t[jslc, var] * IFFT_x(rows) )
 // grid: (ceil(ny/LPB), nwave); block: T*LPB
 // ---------------------------------------------------------------------------
-template <int N>
-__global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_pass(
+template <int N, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
     float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
     long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
     const float2* __restrict__ twg) {
-  using C_ = Cfg<N>;
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH>;
@@ -82,7 +99,9 @@ __global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_pass(
     });
   }
   __syncthreads();
-  LineFFT<N, T, +1, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  // rows lie inside one wavefront when T divides 64: wave-level barriers suffice
+  using RowSync = std::conditional_t<(64 % T == 0), SyncWave, SyncBlock>;
+  LineFFT<N, T, +1, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -95,7 +114,7 @@ __global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_pass(
       }
     });
   }
-  LineFFT<N, T, -1, P, true, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  LineFFT<N, T, -1, P, true, L, NREG>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -172,11 +191,14 @@ struct ColArgs {
   long long chan_wstride;   // per-wave stride of the channels (items)
   long long chan_off;       // plane offset in the channels (items)
   const float2* twg;
+  // column-major copies used by the wave-local column kernel: [x][y]
+  const float2* pro_t;      // propagator of this slice
+  const float* detw_t;      // [ndet][nx][ny] detector weights
+  const float* cppl_t;      // constant phase plate
 };
 
-template <int N, bool INIT, bool EXIT>
-__global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_pass(ColArgs a) {
-  using C_ = Cfg<N>;
+template <int N, bool INIT, bool EXIT, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
   using P = typename C_::P;
   constexpr int T = C_::TC, C = C_::C;
   using L = TileLayout<N, C, C_::PADSH>;
@@ -262,21 +284,33 @@ __global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_pass(ColArgs a) 
     for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
     if (valid) {
       const long long n = (long long)N * nx;
-      static_for<0, M1::NB>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        const int b = t + i * T;
-        if (M1::FULL || b < M1::NBF) {
+      for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
+        const float* dw = a.detw + d * n + x;
+        float wt[NREG];
+        static_for<0, M1::NB>([&](auto I) {  // all weight loads of this detector first
+          constexpr int i = decltype(I)::value;
+          const int b = t + i * T;
           static_for<0, M1::R0>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            const long long idx = (long long)(b + q * M1::NBF) * nx + x;
-            const float2 z = v[i * M1::R0 + q];
-            const float p = z.x * z.x + z.y * z.y;
-#pragma unroll
-            for (int d = 0; d < kMaxDet; ++d)
-              if (d < a.ndet) acc[d] += p * a.detw[d * n + idx];
+            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF) ? dw[(long long)(b + q * M1::NBF) * nx] : 0.f;
           });
-        }
-      });
+        });
+        float sacc = 0.f;
+        static_for<0, M1::NB>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          const int b = t + i * T;
+          if (M1::FULL || b < M1::NBF) {
+            static_for<0, M1::R0>([&](auto Q) {
+              constexpr int q = decltype(Q)::value;
+              const float2 z = v[i * M1::R0 + q];
+              sacc += (z.x * z.x + z.y * z.y) * wt[i * M1::R0 + q];
+            });
+          }
+        });
+#pragma unroll
+        for (int dd = 0; dd < kMaxDet; ++dd)
+          if (dd == d) acc[dd] += sacc;
+      }
     }
     block_reduce_det(acc, a.ndet, (float*)s_buf,
                      a.partial + w * a.part_wstride + a.part_off + blockIdx.x, a.part_dstride);
@@ -334,14 +368,237 @@ __global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_pass(ColArgs a) 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// column pass, wave-local variant (same arguments and results as k_col_pass).
+// The tile of CW adjacent columns is staged through LDS with coalesced
+// row-segment loads and stores; each column is transformed inside one
+// wavefront (TW threads, TW divides 64), so the FFT exchanges need only
+// wave-level barriers.  Propagator, detector weights and phase plate are
+// read from column-major copies (pro_t, detw_t, cppl_t) so that the lanes of
+// a column access them contiguously.
+// grid: (ceil(nx/CW), nwave); block: TW*CW
+// ---------------------------------------------------------------------------
+template <int N, bool INIT, bool EXIT, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
+  using P = typename C_::P;
+  constexpr int T = C_::TW, C = C_::CW, BS = T * C;
+  static_assert(64 % T == 0, "a column must lie inside one wavefront");
+  static_assert(BS % 64 == 0, "block must be whole wavefronts");
+  using L = ColLayout<N, C_::PADSH>;
+  constexpr int NREG = regs_needed<N, T, P>();
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(C)];
+  __shared__ float s_red[(BS / 64) * kMaxDet];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int c = wv * (64 / T) + lane / T;  // column within the tile
+  const int t = lane % T;
+  for (int e = tid; e < N; e += BS) s_tw[e] = a.twg[e];
+  const long long w = blockIdx.y;
+  const int nx = a.nx;
+  const long long n = (long long)N * nx;
+  const int x0 = blockIdx.x * C;
+  const int ncol = (nx - x0 < C) ? (nx - x0) : C;
+  const int x = x0 + c;
+  const bool valid = c < ncol;
+  float2* wv_base = a.wave + w * a.wave_stride + x0;
+  // staging sweep: KST elements per thread, all loads issued before the LDS
+  // writes (columns past the grid edge read column x0 and are discarded)
+  constexpr int KST = (N * C + BS - 1) / BS;
+  constexpr bool STFULL = (N * C) % BS == 0;
+  {
+    const float2* src = INIT ? (a.src + w * a.src_stride + x0) : wv_base;
+    float2 st[KST];
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if (STFULL || i < N * C) {
+        const int e = i / C, cc = i % C;
+        st[k] = src[e * nx + (cc < ncol ? cc : 0)];
+      }
+    });
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if (STFULL || i < N * C) s_buf[L::idx(i % C, i / C)] = st[k];
+    });
+  }
+  __syncthreads();
+  using M0 = IOMap<N, T, P, false>;
+  using M1 = IOMap<N, T, P, true>;
+  const long long colb = (long long)(valid ? x : x0) * N;  // column-major offset
+  float2 v[NREG];
+  if constexpr (INIT) {
+    float dx = 0.f, dy = 0.f, fz = 0.f;
+    const bool pp = a.ppar != nullptr;
+    if (pp) {
+      const float4 q4 = a.ppar[w];
+      dx = q4.x; dy = q4.y; fz = q4.z;
+    }
+    const float kx = (float)fidx(valid ? x : 0, nx) * a.kscx;
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF) {
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          const int e = b + q * M1::NBF;
+          float2 val = s_buf[L::idx(c, e)];
+          if (pp && valid) {
+            const float ky = (float)fidx(e, N) * a.kscy;
+            const float ksqr = kx * kx + ky * ky;
+            float chi = -6.28318531f * (dx * kx + dy * ky + fz * ksqr);
+            if (a.cppl_t) chi -= a.cppl_t[colb + e];
+            float sn, cs;
+            sincosf(chi, &sn, &cs);
+            val = cmul(val, make_float2(cs, sn));
+          }
+          v[i * M1::R0 + q] = val;
+        });
+      }
+    });
+  } else {
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF) {
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M0::R0 + q] = s_buf[L::idx(c, b + q * M0::NBF)];
+        });
+      }
+    });
+    SyncWave{}();
+    LineFFT<N, T, -1, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    const float2* pcol = a.pro_t + colb;
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF) {
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * M1::R0 + q] = cmul(v[i * M1::R0 + q], pcol[b + q * M1::NBF]);
+        });
+      }
+    });
+  }
+  if (a.detect) {
+    float acc[kMaxDet];
+#pragma unroll
+    for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
+    if (valid) {
+      float pw[NREG];
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          const float2 z = v[i * M1::R0 + q];
+          pw[i * M1::R0 + q] = z.x * z.x + z.y * z.y;
+        });
+      });
+      for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
+        const float* dw = a.detw_t + d * n + colb;
+        float wt[NREG];
+        static_for<0, M1::NB>([&](auto I) {  // all weight loads of this detector first
+          constexpr int i = decltype(I)::value;
+          const int b = t + i * T;
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF) ? dw[b + q * M1::NBF] : 0.f;
+          });
+        });
+        float sacc = 0.f;
+        static_for<0, M1::NB>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          const int b = t + i * T;
+          if (M1::FULL || b < M1::NBF) {
+            static_for<0, M1::R0>([&](auto Q) {
+              constexpr int q = decltype(Q)::value;
+              sacc += pw[i * M1::R0 + q] * wt[i * M1::R0 + q];
+            });
+          }
+        });
+#pragma unroll
+        for (int dd = 0; dd < kMaxDet; ++dd)
+          if (dd == d) acc[dd] += sacc;
+      }
+    }
+    block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + blockIdx.x,
+                     a.part_dstride);
+    if (valid && (a.dif || a.wff)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
+      const float wg = a.cwgt[w];
+      const long long cb = w * a.chan_wstride + a.chan_off;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
+            const float2 z = v[i * M1::R0 + q];
+            if (a.dif) a.dif[idx] += wg * (z.x * z.x + z.y * z.y);
+            if (a.wff) {
+              float2 o = a.wff[idx];
+              o.x += z.x * wg;
+              o.y += z.y * wg;
+              a.wff[idx] = o;
+            }
+          });
+        }
+      });
+    }
+  }
+  if constexpr (EXIT) {
+    SyncWave{}();
+    static_for<0, M1::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M1::FULL || b < M1::NBF) {
+        static_for<0, M1::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          s_buf[L::idx(c, b + q * M1::NBF)] = v[i * M1::R0 + q];
+        });
+      }
+    });
+  } else {
+    SyncWave{}();
+    LineFFT<N, T, +1, P, true, L, NREG>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    SyncWave{}();
+    static_for<0, M0::NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if (M0::FULL || b < M0::NBF) {
+        static_for<0, M0::R0>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          s_buf[L::idx(c, b + q * M0::NBF)] = v[i * M0::R0 + q];
+        });
+      }
+    });
+  }
+  __syncthreads();
+  {  // store the tile: all LDS reads first, then the (guarded) global stores
+    float2 st[KST];
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if (STFULL || i < N * C) st[k] = s_buf[L::idx(i % C, i / C)];
+    });
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if ((STFULL || i < N * C) && i % C < ncol) wv_base[(i / C) * nx + i % C] = st[k];
+    });
+  }
+}
+
 // ---------------------------------------------------------------------------
 // standalone line transforms (2-D FFT = row transform + column transform)
 // ---------------------------------------------------------------------------
-template <int N, int DIR>
-__global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_fft(float2* __restrict__ data,
+template <int N, int DIR, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::T * C_::LPB) void k_row_fft(float2* __restrict__ data,
                                                                       int ny, long long stride,
                                                                       const float2* __restrict__ twg) {
-  using C_ = Cfg<N>;
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH>;
@@ -382,11 +639,10 @@ __global__ __launch_bounds__(Cfg<N>::T * Cfg<N>::LPB) void k_row_fft(float2* __r
   }
 }
 
-template <int N, int DIR>
-__global__ __launch_bounds__(Cfg<N>::TC * Cfg<N>::C) void k_col_fft(float2* __restrict__ data,
+template <int N, int DIR, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::TC * C_::C) void k_col_fft(float2* __restrict__ data,
                                                                      int nx, long long stride,
                                                                      const float2* __restrict__ twg) {
-  using C_ = Cfg<N>;
   using P = typename C_::P;
   constexpr int T = C_::TC, C = C_::C;
   using L = TileLayout<N, C, C_::PADSH>;
@@ -454,11 +710,23 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   a.dif = h.dif; a.wff = (float2*)h.wff; a.cwgt = h.cwgt; a.chan_wstride = h.chan_wstride;
   a.chan_off = h.chan_off;
   a.twg = (const float2*)h.twy;
-  dim3 g((h.nx + Cfg<N>::C - 1) / Cfg<N>::C, nwave), b(Cfg<N>::TC * Cfg<N>::C);
-  if (init && exit_) hipLaunchKernelGGL((k_col_pass<N, true, true>), g, b, 0, st, a);
-  else if (init) hipLaunchKernelGGL((k_col_pass<N, true, false>), g, b, 0, st, a);
-  else if (exit_) hipLaunchKernelGGL((k_col_pass<N, false, true>), g, b, 0, st, a);
-  else hipLaunchKernelGGL((k_col_pass<N, false, false>), g, b, 0, st, a);
+  a.pro_t = (const float2*)h.pro_t;
+  a.detw_t = h.detw_t;
+  a.cppl_t = h.cppl_t;
+  using CF = Cfg<N>;
+  if constexpr (CF::WL) {
+    dim3 g((h.nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_w<N, false, false>), g, b, 0, st, a);
+  } else {
+    dim3 g((h.nx + CF::C - 1) / CF::C, nwave), b(CF::TC * CF::C);
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass<N, true, true>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass<N, true, false>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass<N, false, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass<N, false, false>), g, b, 0, st, a);
+  }
   return hipGetLastError();
 }
 

@@ -1,0 +1,198 @@
+"""Pins the oracle's restated setup functions and readout arithmetic to
+closed forms, library routines and known-answer tests:
+  * wavelength (JMultiSlice.cpp:372-378) vs the relativistic formula with
+    scipy.constants;
+  * propagators (JMultiSlice.cpp:998-1106): modulus 1/n inside the 2/3 band
+    limit, 0 outside, phase -pi lambda dz k^2 (Fresnel), planar = Fresnel to
+    second order;
+  * ring detectors (:1274-1438): half-open rings, disjoint rings add up to a
+    disk, segments partition a ring, mask lists exactly the active pixels;
+  * probe (JProbeGen.cpp:707-777): unit norm, aperture radius;
+  * detection planes (:579-627): enumerated cases;
+  * fdncs2m butterfly sum and masked dot products (:38-242, :3279-3330): exact
+    on integer data, float-accurate on random data;
+  * RNG (rng.cpp): the MT19937 known answer of the C++ standard
+    ([rand.predef]: 10000th output of the default-seeded engine is
+    4123659995), unirand's closed form, the polar-method normal deviates
+    against numpy's normal_distribution-independent moments;
+  * variant sequences (:1571-1676): permutation windows, no repeat across a
+    window boundary, single-variant slices give 0.
+"""
+import numpy as np
+import pytest
+import scipy.constants as sc
+
+from oracle import oracle as O
+
+
+def test_wavelength_closed_form():
+    for ht in (80.0, 200.0, 300.0):
+        E = ht * 1e3 * sc.e
+        mc2 = sc.m_e * sc.c ** 2
+        lam = sc.h * sc.c / np.sqrt(E * (E + 2 * mc2)) * 1e9  # nm
+        assert abs(O.wavelength(ht) - lam) / lam < 1e-6
+    assert abs(O.wavelength(200.0) - 0.0025079) < 1e-7
+
+
+def _k2(nx, ny, a0x, a0y):
+    fx = ((np.arange(nx) + nx // 2) % nx - nx // 2) / a0x
+    fy = ((np.arange(ny) + ny // 2) % ny - ny // 2) / a0y
+    return fx[None, :] ** 2 + fy[:, None] ** 2, fx, fy
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (96, 64), (65, 33)])
+def test_propagator_fresnel_closed_form(nx, ny):
+    a0x, a0y, wl, dz = 1.7, 1.3, 0.0025079, 0.19525
+    P = O.propagator(nx, ny, a0x, a0y, wl, dz, 0.0, 0.0, 1)
+    k2, fx, fy = _k2(nx, ny, a0x, a0y)
+    gmax = min((nx - nx % 2) / 2 / a0x, (ny - ny % 2) / 2 / a0y) * 0.666666666666667
+    inband = k2 <= gmax ** 2
+    n = nx * ny
+    assert np.allclose(np.abs(P[inband]) * n, 1.0, atol=1e-6)
+    assert np.all(P[~inband] == 0)
+    ref = np.exp(-1j * np.pi * wl * dz * k2) / n
+    assert np.abs(P[inband] - ref[inband]).max() * n < 2e-6
+
+
+def test_propagator_planar_vs_fresnel_small_angle():
+    nx = ny = 64
+    a0 = 20.0  # large cell -> small angles
+    wl, dz = 0.0025079, 0.2
+    Pp = O.propagator(nx, ny, a0, a0, wl, dz, 0.0, 0.0, 0)
+    Pf = O.propagator(nx, ny, a0, a0, wl, dz, 0.0, 0.0, 1)
+    assert np.abs(Pp - Pf).max() * nx * ny < 1e-4
+
+
+def test_propagator_tilt_is_shift():
+    """A tilted Fresnel propagator = untilted one times a linear phase (slices shift)."""
+    nx = ny = 48
+    a0, wl, dz = 1.5, 0.0025079, 0.2
+    tx = 0.3  # deg
+    Pt = O.propagator(nx, ny, a0, a0, wl, dz, tx, 0.0, 1)
+    P0 = O.propagator(nx, ny, a0, a0, wl, dz, 0.0, 0.0, 1)
+    k2, fx, fy = _k2(nx, ny, a0, a0)
+    t = np.deg2rad(tx)
+    ph = np.exp(-1j * (dz * np.pi / wl * np.cos(t) * (-2 * wl * fx[None, :] * t)))
+    m = P0 != 0
+    assert np.abs(Pt[m] - (P0 * ph)[m]).max() * nx * ny < 5e-5
+
+
+def test_ring_detectors():
+    nx = ny = 128
+    a0, wl = 3.124, 0.0025079
+    k2, fx, fy = _k2(nx, ny, a0, a0)
+    g = np.sqrt(k2)
+    k0 = 1 / wl
+    rings = [(0, 11), (11, 22), (22, 40)]
+    tot = np.zeros((ny, nx))
+    for b0, b1 in rings:
+        det, msk = O.ring_detector(nx, ny, a0, a0, wl, 1, b0, b1)
+        ref = ((g >= b0 * 1e-3 * k0) & (g < b1 * 1e-3 * k0)).astype(np.float32)
+        assert np.array_equal(det, ref)
+        assert np.array_equal(np.sort(msk), np.flatnonzero(ref.ravel()))
+        tot += det
+    disk, _ = O.ring_detector(nx, ny, a0, a0, wl, 1, 0, 40)
+    assert np.array_equal(tot, disk)
+    # four 90-degree segments partition the ring
+    seg = np.zeros((ny, nx))
+    for p0 in (0, 90, 180, 270):
+        d, _ = O.ring_detector(nx, ny, a0, a0, wl, 1, 11, 22, p0, p0 + 90)
+        seg += d
+    full, _ = O.ring_detector(nx, ny, a0, a0, wl, 1, 11, 22)
+    assert np.array_equal(seg, full)
+    # first moment pattern: sensitivity kx on the ring
+    dk, _ = O.ring_detector(nx, ny, a0, a0, wl, 1, 11, 22, kmom1=1)
+    assert np.allclose(dk, full * fx[None, :], atol=1e-4)
+
+
+def test_probe_norm_and_aperture():
+    nx = ny = 128
+    a0, wl = 3.124, 0.0025079
+    for rs in (0.0, 0.03):
+        p = O.probe_wave(nx, ny, a0, a0, wl, 25.0, rs)
+        assert abs(np.sum(np.abs(p) ** 2) - 1.0) < 1e-5
+    p = O.probe_wave(nx, ny, a0, a0, wl, 25.0, 0.0)
+    k2, _, _ = _k2(nx, ny, a0, a0)
+    qlim = 25e-3 / wl
+    inside = np.sqrt(k2) < qlim
+    inside[0, 0] = True
+    assert np.array_equal(p != 0, inside)
+    assert np.all(np.abs(p.imag) == 0)
+
+
+@pytest.mark.parametrize("ndetper,nobj,expect", [
+    (0, 10, {10: 0}),
+    (3, 10, {0: 0, 3: 1, 6: 2, 9: 3, 10: 4}),
+    (5, 10, {0: 0, 5: 1, 10: 2}),
+    (1, 3, {0: 0, 1: 1, 2: 2, 3: 3}),
+    (4, 0, {0: 0}),
+])
+def test_detection_planes(ndetper, nobj, expect):
+    h, n = O.detection_planes(ndetper, nobj)
+    assert n == len(expect)
+    for i in range(nobj + 1):
+        assert h[i] == expect.get(i, -1)
+
+
+def test_fdncs2m_sums():
+    for n in (1, 2, 3, 31, 32, 33, 1000, 4096, 4097, 10000, 70000):
+        a = np.arange(n, dtype=np.float32) % 7  # exact in float
+        assert O.fdncs2m(a) == float(a.astype(np.int64).sum())
+    rng = np.random.default_rng(1)
+    for n in (100, 5000, 300000):
+        a = rng.random(n).astype(np.float32)
+        s = O.fdncs2m(a)
+        assert abs(s - a.astype(np.float64).sum()) / a.sum() < 1e-6
+
+
+def test_masked_dot():
+    rng = np.random.default_rng(2)
+    for n in (1, 2, 7, 1001, 50000):
+        m = rng.choice(100000, n, replace=False).astype(np.int32)
+        a = rng.random(100000).astype(np.float32)
+        b = rng.random(100000).astype(np.float32)
+        ref = float(np.sum(a[m].astype(np.float64) * b[m]))
+        assert abs(O.masked_dot(m, a, b) - ref) / ref < 2e-6
+
+
+def test_mt19937_known_answer():
+    r = O.Rng(5489)
+    v = [r.raw() for _ in range(10000)]
+    assert v[-1] == 4123659995
+    r2 = O.Rng(5489)
+    assert r2.rand() == v[0] >> 1
+
+
+def test_unirand_closed_form():
+    r, q = O.Rng(42), O.Rng(42)
+    for _ in range(1000):
+        x = q.raw() >> 1
+        assert r.unirand(2.0, 10.0) == 2.0 + (x / 2147483647.0) * 8.0
+
+
+def test_normrand_moments():
+    r = O.Rng(1234)
+    x = np.array([r.normrand(0.0, 1.0) for _ in range(200000)])
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1.0) < 0.01
+    # N(0,1) quantile check
+    from scipy.stats import kstest
+    assert kstest(x, "norm").pvalue > 1e-3
+
+
+def test_variant_sequence_properties():
+    nv = [4, 1, 6]
+    obj = np.array([i % 3 for i in range(300)], np.int32)
+    seqs = O.variant_sequences(17, nv, obj, 5)
+    for s in seqs:
+        for t in range(3):
+            v = s[obj == t]
+            if nv[t] < 2:
+                assert np.all(v == 0)
+                continue
+            win = v[: (len(v) // nv[t]) * nv[t]].reshape(-1, nv[t])
+            for w in win:
+                assert sorted(w) == list(range(nv[t]))
+            assert np.all(win[1:, 0] != win[:-1, -1])  # no repeat across a window boundary
+    # different seeds give different sequences, same seed the same
+    assert not np.array_equal(seqs, O.variant_sequences(18, nv, obj, 5))
+    assert np.array_equal(seqs, O.variant_sequences(17, nv, obj, 5))

@@ -1,0 +1,150 @@
+"""Host-side parts of the product library (libjmultislice_hip.so) against
+the oracle, without a GPU: setup calculators through the flat C-ABI, the
+specimen builder and the variant sequencer must agree bit for bit (they are
+restated from the same reference functions and evaluated in the same
+precision); the C-ABI must export every entry point its headers declare."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import jmultislicelib_amd as J
+from jmultislicelib_amd import JMultiSlice, configs
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"//[^\n]*", "", txt)
+    names = set()
+    for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt):
+        name = m.group(1)
+        if name not in ("if", "while", "for", "return", "sizeof"):
+            names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", ["JMultiSliceLib.h", "jms_hip.h", "jms_specimen.h", "jms_sequence.h"])
+def test_abi_exports_every_declared_symbol(header):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", J.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    declared = _declared(header)
+    assert declared, header
+    missing = sorted(declared - exported)
+    assert not missing, missing
+    # the Python binding covers them too
+    assert not sorted(declared - set(J.EXPORTED))
+
+
+def test_library_loads():
+    L = J.lib()
+    assert abs(L.GetJMSVersion() - 0.46) < 1e-6
+
+
+def _setup(nx=64, ny=48, a=(1.7, 1.3), ht=200.0):
+    M = JMultiSlice()
+    M.SetHighTension(ht)
+    M.SetGridSize(nx, ny)
+    M.SetSupercellSize(a[0], a[1], 0.2)
+    return M
+
+
+def test_wavelength():
+    M = _setup()
+    assert M.GetWaveLength() == O.wavelength(200.0)
+
+
+@pytest.mark.parametrize("ntype,otx,oty", [(1, 0.0, 0.0), (0, 0.0, 0.0), (1, 0.4, -0.2), (0, 0.3, 0.5)])
+def test_propagator_bit_exact(ntype, otx, oty):
+    M = _setup()
+    P = M.CalculatePropagator(0.19525, otx, oty, ntype)
+    R = O.propagator(64, 48, 1.7, 1.3, M.GetWaveLength(), 0.19525, otx, oty, ntype)
+    assert np.array_equal(P.view(np.uint32), R.view(np.uint32))
+
+
+@pytest.mark.parametrize("ntype", [0, 1])
+@pytest.mark.parametrize("ring", [(0, 11, 0, 0, 0, 0), (11, 22, 0, 0, 0, 0), (70, 200, 0, 0, 0, 0),
+                                  (10, 40, 30, 120, 0, 0), (10, 40, 300, 40, 0, 0), (5, 30, 0, 0, 3.0, -2.0)])
+def test_ring_detector_bit_exact(ntype, ring):
+    M = _setup()
+    M.CalculatePropagator(0.2, 0, 0, ntype)  # selects the angle mapping
+    b0, b1, p0, p1, cx, cy = ring
+    for kmom in (0, 1, 2):
+        det, msk = M.CalculateRingDetector(b0, b1, p0, p1, cx, cy, kmom1=kmom)
+        rd, rm = O.ring_detector(64, 48, 1.7, 1.3, M.GetWaveLength(), ntype, b0, b1, p0, p1, cx, cy, kmom1=kmom)
+        assert np.array_equal(det, rd)
+        assert np.array_equal(msk, rm)
+
+
+def test_ring_detector_profile_file(tmp_path):
+    M = _setup()
+    M.CalculatePropagator(0.2, 0, 0, 1)
+    prof = np.linspace(1.0, 0.2, 40).astype(np.float32)
+    f = tmp_path / "prof.txt"
+    f.write_text(f"{len(prof)}, 10.5\n" + "\n".join(f"{v:.7f}" for v in prof) + "\n")
+    det, msk = M.CalculateRingDetector(15, 60, sdsprofile=str(f))
+    rd, rm = O.ring_detector(64, 48, 1.7, 1.3, M.GetWaveLength(), 1, 15, 60,
+                             profile=np.array([float(f"{v:.7f}") for v in prof], np.float32), refpix=10.5)
+    assert np.array_equal(msk, rm)
+    assert np.array_equal(det, rd)
+
+
+@pytest.mark.parametrize("alpha,rs,asym,adir,bt", [(25.0, 0.03, 0, 0, (0, 0)), (25.0, 0.0, 0, 0, (0, 0)),
+                                                   (18.0, 0.05, 0.1, 30.0, (1.5, -2.0))])
+def test_probe_bit_exact(alpha, rs, asym, adir, bt):
+    M = _setup()
+    p = M.CalculateProbeWaveFourier(alpha, rs, 0.0, 0.0, asym, adir, bt[0], bt[1])
+    r = O.probe_wave(64, 48, 1.7, 1.3, M.GetWaveLength(), alpha, rs, asym, adir, 0.0, 0.0, bt[0], bt[1])
+    assert np.array_equal(p.view(np.uint32), r.view(np.uint32))
+
+
+def test_unscramble_hash():
+    M = _setup()
+    h = M.GetUnscrambleHash()
+    a = np.arange(64 * 48).reshape(48, 64)
+    assert np.array_equal(a.ravel()[h].reshape(48, 64), np.fft.fftshift(a))
+
+
+@pytest.mark.parametrize("ndetper,nobj", [(0, 10), (3, 10), (5, 10), (1, 3), (4, 0), (7, 200)])
+def test_detection_planes(ndetper, nobj):
+    h, n = configs.detection_planes(ndetper, nobj)
+    r, m = O.detection_planes(ndetper, nobj)
+    assert n == m and np.array_equal(h, r)
+
+
+def test_variant_sequences_bit_exact():
+    sp = configs.perovskite_setup(n=32, ncell=2, nslices=40, nvar=6)
+    a = configs.variant_sequences(sp, 50, seed=42)
+    b = O.variant_sequences(42, sp.nvarslc, sp.objslc, 50)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("t", [0, 1])
+def test_frozen_phonon_sites_and_gratings_bit_exact(t):
+    for v in range(3):
+        xy, z = configs.perovskite_sites(3, t, 8, v, 0.08, 1000)
+        oxy, oz = O.perovskite_sites(3, t, 0.08, 1000 + 8 * t + v, 1)
+        assert np.array_equal(xy, oxy) and np.array_equal(z, oz)
+        g = configs.gaussian_grating(48, 3 * 3.905, xy, z)
+        og = O.gaussian_grating(48, 48, 3 * 3.905, 3 * 3.905, oxy, oz)
+        assert np.array_equal(g.view(np.uint32), og.view(np.uint32))
+
+
+def test_frozen_phonon_displacement_statistics():
+    xy0, _ = configs.perovskite_sites(8, 1, 1, 0, 0.08, 1000)   # undisplaced
+    d = []
+    for v in range(8):
+        xy, _ = configs.perovskite_sites(8, 1, 8, v, 0.08, 1000)
+        d.append(xy - xy0)
+    d = np.concatenate(d)
+    assert abs(d.std() - 0.08) < 0.005 and abs(d.mean()) < 0.005
+
+
+def test_cpu_code_is_rejected():
+    M = _setup()
+    assert M.PhaseGratingSetup(1, 64, 48, 1, 1, [1]) != 0
