@@ -40,10 +40,11 @@ def log(*a):
 
 
 def cpu_baseline(sp, dx, dy, var, npass, cores):
-    """Oracle float32 scan on `cores` threads: 2*cores probe passes (one
-    position each), scaled to positions/s of npass passes."""
+    """Oracle float32 scan on `cores` threads: 12*cores probe passes (one
+    position each, about 10 s of CPU work), scaled to positions/s of npass
+    passes."""
     from oracle import oracle as O
-    nsample = 2 * cores
+    nsample = 12 * cores
     t0 = time.perf_counter()
     O.stem_scan(sp, dx[:nsample], dy[:nsample], [1.0], var[:nsample], prec=32, nthreads=cores)
     dt = time.perf_counter() - t0
@@ -62,7 +63,21 @@ def main():
     ap.add_argument("--batch", type=int, default=2048, help="probe passes resident per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay batches from a HIP graph (no per-kernel timing)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference "
+                         "kernel for scaling HBM counter readings) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        nel = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 copies of {nel * 4} bytes (read {nel * 4} + write {nel * 4} per copy)")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "

@@ -39,6 +39,22 @@ struct Cfg {
   // columns per block of the column kernel in use (partial-sum tiles)
   static constexpr int CC = WL ? CW : C;
   static constexpr int PADSH = 3;
+  // minimum waves per SIMD requested from the register allocator for the
+  // wave-local column kernel (1 = no constraint).  At N = 512 eight waves let
+  // two 1024-thread blocks share a CU, so one block's FFTs overlap the other's
+  // memory phase (measured 1.35 -> 1.11 ms per 1024 waves on MI355X).
+  static constexpr int WPE_COL = (N == 512) ? 8 : 1;
+  static constexpr int WPE_ROW = 1;
+  // nontemporal wave loads/stores in the row / column pass: the streamed
+  // waves then leave the reused phase gratings and propagators in L2
+  // (row pass 0.84 -> 0.78 ms per 1024 waves at N = 512)
+  static constexpr bool NT_ROW = N >= 256;
+  static constexpr bool NT_COL = N >= 256;
+  // LDS position policy of line layouts (0 padded, 1 swizzled) and twiddle
+  // powers by product tree (one table read per butterfly) instead of one
+  // table read per leg
+  static constexpr int SW = 0;
+  static constexpr bool TWR = true;
 };
 
 

@@ -100,6 +100,30 @@ __device__ __forceinline__ float2 cmulconj(float2 a, float2 b) {  // a * conj(b)
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// global loads/stores with the nontemporal hint: streamed data then does not
+// displace reused arrays (phase gratings, propagators) from L2
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ float2 ldg(const float2* p) {
+  if constexpr (NT) {
+    const f2v r = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    return make_float2(r.x, r.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void stg(float2* p, float2 v) {
+  if constexpr (NT) {
+    f2v r;
+    r.x = v.x;
+    r.y = v.y;
+    __builtin_nontemporal_store(r, reinterpret_cast<f2v*>(p));
+  } else {
+    *p = v;
+  }
+}
 // multiply by sigma*i, sigma = +1 (DIR > 0) or -1 (DIR < 0)
 template <int DIR>
 __device__ __forceinline__ float2 mul_si(float2 a) {
@@ -340,21 +364,65 @@ constexpr int plan_product() {
 
 // ---------------------------------------------------------------------------
 // LDS access policy for a line.  Element e of line `l` lives at
-//   ROWMAJOR: lds[l * STRIDE + e + (e >> PADSH)]
-//   interleaved (columns): lds[(e + (e >> PADSH)) * LINES + l]
+//   ROWMAJOR: lds[l * STRIDE + pos(e)]
+//   interleaved (columns): lds[pos(e) * LINES + l]
+// with pos(e) one of
+//   padded   (SW = 0): e + (e >> PADSH), one spare slot per 2^PADSH elements;
+//   swizzled (SW = 1): e ^ h(e >> 5), an XOR of the low five bits by
+//            h(r) = (r & 7) | ((r >> 1) & 3) << 3.
+// gfx950 serves 8-byte LDS accesses 32 lanes per cycle over 64 four-byte
+// banks, so an access is conflict free when the 32 lanes' 8-byte word
+// addresses are distinct mod 32.  For the Stockham exchanges of a 64-thread
+// line (reads of consecutive elements, writes at stride R*s) the swizzle is
+// conflict free for power-of-two radix-8/16 plans, the padding is better for
+// mixed radices; the per-size choice is made in Cfg.
+//
+// step(d) (padded policy only) is the index distance from element e0 to
+// element e0 + d of the same line when the number of padding slots between
+// them is fixed at compile time, i.e. when d is a multiple of the padding
+// period 2^PADSH or e0 lies on a period boundary; then idx(l, e0 + d) =
+// idx(l, e0) + step(d) and a thread's accesses share one base address with
+// immediate offsets.
 // ---------------------------------------------------------------------------
-template <int N, int PADSH>
-struct LineLayout {  // one line per group of threads, contiguous
-  static constexpr int STRIDE = N + (N >> PADSH) + 1;
-  __device__ static __forceinline__ int idx(int line, int e) { return line * STRIDE + e + (e >> PADSH); }
+template <int SW, int PADSH>
+struct LdsPos {
+  static constexpr bool AFFINE = (SW == 0);
+  static constexpr int PAD = PADSH;
+  __device__ static __forceinline__ int pos(int e) {
+    if constexpr (SW == 0) {
+      return e + (e >> PADSH);
+    } else {
+      const int r = e >> 5;
+      return e ^ ((r & 7) | (((r >> 1) & 3) << 3));
+    }
+  }
+  // slots a line of n elements occupies
+  static constexpr int span(int n) { return SW == 0 ? n + (n >> PADSH) + 1 : ((n + 31) / 32) * 32; }
+  static constexpr int step(int d) { return d + (d >> PADSH); }
+};
+
+template <int N, int PADSH, int SW = 0>
+struct LineLayout : LdsPos<SW, PADSH> {  // one line per group of threads, contiguous
+  using B = LdsPos<SW, PADSH>;
+  static constexpr int STRIDE = B::span(N);
+  __device__ static __forceinline__ int idx(int line, int e) { return line * STRIDE + B::pos(e); }
   static constexpr int words(int lines) { return lines * STRIDE; }
 };
 template <int N, int LINES, int PADSH>
-struct TileLayout {  // LINES lines interleaved (line fastest)
+struct TileLayout : LdsPos<0, PADSH> {  // LINES lines interleaved (line fastest)
   static constexpr int ROWS = N + (N >> PADSH) + 1;
   __device__ static __forceinline__ int idx(int line, int e) { return (e + (e >> PADSH)) * LINES + line; }
+  static constexpr int step(int d) { return (d + (d >> PADSH)) * LINES; }
   static constexpr int words(int) { return ROWS * LINES; }
 };
+
+// element b + q*NBF of line `line` for q = 0..R-1 (the register slots of one
+// butterfly), by a shared base when the layout allows it
+template <class Layout, int NBF, int Q>
+__device__ __forceinline__ int lds_slot(int line, int b, int a0) {
+  if constexpr (Layout::AFFINE && NBF % (1 << Layout::PAD) == 0) return a0 + Layout::step(Q * NBF);
+  else return Layout::idx(line, b + Q * NBF);
+}
 
 // ---------------------------------------------------------------------------
 // Line FFT driver.
@@ -365,7 +433,7 @@ struct TileLayout {  // LINES lines interleaved (line fastest)
 //   tw  : W_N^e = exp(-2 pi i e/N), e in [0, N) (forward twiddles)
 //   SYNC: callable issuing the barrier appropriate for the line group
 // ---------------------------------------------------------------------------
-template <int N, int T, int DIR, class P, bool REV, class Layout, int NREG, int S = 0>
+template <int N, int T, int DIR, class P, bool REV, class Layout, int NREG, bool TWR = false, int S = 0>
 struct LineFFT {
   template <class Sync>
   static __device__ __forceinline__ void run(float2 (&v)[NREG], float2* lds, int line,
@@ -382,11 +450,26 @@ struct LineFFT {
         DFT<R, DIR>::run(&v[i * R]);
         if constexpr (!LAST) {
           const int base = (b / s) * s;
-          static_for<1, R>([&](auto K) {
-            constexpr int k = decltype(K)::value;
-            const float2 w = tw[base * k];
-            v[i * R + k] = (DIR < 0) ? cmul(v[i * R + k], w) : cmulconj(v[i * R + k], w);
-          });
+          if constexpr (TWR) {
+            // one table read, W^(base*k) for k >= 2 by a product tree of
+            // depth <= log2(k): w^k = w^(k/2) * w^(k - k/2)
+            float2 pw[R];
+            pw[1] = tw[base];
+            static_for<2, R>([&](auto K) {
+              constexpr int k = decltype(K)::value;
+              pw[k] = cmul(pw[k / 2], pw[k - k / 2]);
+            });
+            static_for<1, R>([&](auto K) {
+              constexpr int k = decltype(K)::value;
+              v[i * R + k] = (DIR < 0) ? cmul(v[i * R + k], pw[k]) : cmulconj(v[i * R + k], pw[k]);
+            });
+          } else {
+            static_for<1, R>([&](auto K) {
+              constexpr int k = decltype(K)::value;
+              const float2 w = tw[base * k];
+              v[i * R + k] = (DIR < 0) ? cmul(v[i * R + k], w) : cmulconj(v[i * R + k], w);
+            });
+          }
         }
       }
     });
@@ -396,9 +479,14 @@ struct LineFFT {
         const int b = t + i * T;
         if ((NB * T == NBF) || b < NBF) {
           const int o0 = (b % s) + (b / s) * s * R;
+          // o0 is on a padding-period boundary when s == 1 and R is a multiple
+          // of the period; s*k is a multiple of it when s is
+          constexpr bool CONST_OFF = Layout::AFFINE && ((s % (1 << Layout::PAD) == 0) || (s == 1 && R % (1 << Layout::PAD) == 0));
+          const int a0 = Layout::idx(line, o0);
           static_for<0, R>([&](auto K) {
             constexpr int k = decltype(K)::value;
-            lds[Layout::idx(line, o0 + s * k)] = v[i * R + k];
+            if constexpr (CONST_OFF) lds[a0 + Layout::step(s * k)] = v[i * R + k];
+            else lds[Layout::idx(line, o0 + s * k)] = v[i * R + k];
           });
         }
       });
@@ -410,14 +498,15 @@ struct LineFFT {
         constexpr int i = decltype(I)::value;
         const int b = t + i * T;
         if ((NB2 * T == NBF2) || b < NBF2) {
+          const int a0 = Layout::idx(line, b);
           static_for<0, R2>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            v[i * R2 + q] = lds[Layout::idx(line, b + q * NBF2)];
+            v[i * R2 + q] = lds[lds_slot<Layout, NBF2, q>(line, b, a0)];
           });
         }
       });
       sync();
-      LineFFT<N, T, DIR, P, REV, Layout, NREG, S + 1>::run(v, lds, line, tw, t, sync);
+      LineFFT<N, T, DIR, P, REV, Layout, NREG, TWR, S + 1>::run(v, lds, line, tw, t, sync);
     }
   }
 };

@@ -44,11 +44,12 @@ struct SyncWave {
 // line-major LDS layout for column tiles: column c at c*STRIDE, element e at
 // e + (e >> PADSH); STRIDE = 2 (mod 32) so that the 16 columns of a staging
 // access (c fastest) fall on distinct bank pairs.
-template <int N, int PADSH>
-struct ColLayout {
-  static constexpr int base = N + (N >> PADSH) + 1;
+template <int N, int PADSH, int SW = 0>
+struct ColLayout : LdsPos<SW, PADSH> {
+  using B = LdsPos<SW, PADSH>;
+  static constexpr int base = B::span(N);
   static constexpr int STRIDE = base + ((34 - (base % 32)) % 32);
-  __device__ static __forceinline__ int idx(int line, int e) { return line * STRIDE + e + (e >> PADSH); }
+  __device__ static __forceinline__ int idx(int line, int e) { return line * STRIDE + B::pos(e); }
   static constexpr int words(int lines) { return lines * STRIDE; }
 };
 
@@ -64,13 +65,13 @@ __device__ __forceinline__ int fidx(int i, int n) {
 // grid: (ceil(ny/LPB), nwave); block: T*LPB
 // ---------------------------------------------------------------------------
 template <int N, class C_ = Cfg<N>>
-__global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
+__global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu(C_::WPE_ROW))) void k_row_pass(
     float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
     long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
     const float2* __restrict__ twg) {
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
-  using L = LineLayout<N, C_::PADSH>;
+  using L = LineLayout<N, C_::PADSH, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   __shared__ float2 s_tw[N];
   __shared__ float2 s_buf[L::words(LPB)];
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          v[i * M0::R0 + q] = wrow[b + q * M0::NBF];
+          v[i * M0::R0 + q] = ldg<C_::NT_ROW>(wrow + b + q * M0::NBF);
         });
       }
     });
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
   __syncthreads();
   // rows lie inside one wavefront when T divides 64: wave-level barriers suffice
   using RowSync = std::conditional_t<(64 % T == 0), SyncWave, SyncBlock>;
-  LineFFT<N, T, +1, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, +1, P, false, L, NREG, C_::TWR>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
       }
     });
   }
-  LineFFT<N, T, -1, P, true, L, NREG>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, -1, P, true, L, NREG, C_::TWR>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_pass(
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          wrow[b + q * M0::NBF] = v[i * M0::R0 + q];
+          stg<C_::NT_ROW>(wrow + b + q * M0::NBF, v[i * M0::R0 + q]);
         });
       }
     });
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
       });
     }
     __syncthreads();
-    LineFFT<N, T, -1, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+    LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncBlock{});
     if (valid) {
       const float2* pcol = a.pro + x;
       static_for<0, M1::NB>([&](auto I) {
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
       });
     }
   } else {
-    LineFFT<N, T, +1, P, true, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncBlock{});
     if (valid) {
       static_for<0, M0::NB>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -379,13 +380,13 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
 // a column access them contiguously.
 // grid: (ceil(nx/CW), nwave); block: TW*CW
 // ---------------------------------------------------------------------------
-template <int N, bool INIT, bool EXIT, class C_ = Cfg<N>>
-__global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(ColArgs a) {
   using P = typename C_::P;
   constexpr int T = C_::TW, C = C_::CW, BS = T * C;
   static_assert(64 % T == 0, "a column must lie inside one wavefront");
   static_assert(BS % 64 == 0, "block must be whole wavefronts");
-  using L = ColLayout<N, C_::PADSH>;
+  using L = ColLayout<N, C_::PADSH, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   __shared__ float2 s_tw[N];
   __shared__ float2 s_buf[L::words(C)];
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
       const int i = tid + k * BS;
       if (STFULL || i < N * C) {
         const int e = i / C, cc = i % C;
-        st[k] = src[e * nx + (cc < ncol ? cc : 0)];
+        st[k] = ldg<C_::NT_COL>(src + e * nx + (cc < ncol ? cc : 0));
       }
     });
     static_for<0, KST>([&](auto K) {
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
         static_for<0, M1::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
           const int e = b + q * M1::NBF;
-          float2 val = s_buf[L::idx(c, e)];
+          float2 val = s_buf[lds_slot<L, M1::NBF, q>(c, b, L::idx(c, b))];
           if (pp && valid) {
             const float ky = (float)fidx(e, N) * a.kscy;
             const float ksqr = kx * kx + ky * ky;
@@ -465,12 +466,12 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          v[i * M0::R0 + q] = s_buf[L::idx(c, b + q * M0::NBF)];
+          v[i * M0::R0 + q] = s_buf[lds_slot<L, M0::NBF, q>(c, b, L::idx(c, b))];
         });
       }
     });
     SyncWave{}();
-    LineFFT<N, T, -1, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
     const float2* pcol = a.pro_t + colb;
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
       }
     });
   }
-  if (a.detect) {
+  if constexpr (DET) {
     float acc[kMaxDet];
 #pragma unroll
     for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
@@ -557,13 +558,13 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
       if (M1::FULL || b < M1::NBF) {
         static_for<0, M1::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          s_buf[L::idx(c, b + q * M1::NBF)] = v[i * M1::R0 + q];
+          s_buf[lds_slot<L, M1::NBF, q>(c, b, L::idx(c, b))] = v[i * M1::R0 + q];
         });
       }
     });
   } else {
     SyncWave{}();
-    LineFFT<N, T, +1, P, true, L, NREG>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
     SyncWave{}();
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          s_buf[L::idx(c, b + q * M0::NBF)] = v[i * M0::R0 + q];
+          s_buf[lds_slot<L, M0::NBF, q>(c, b, L::idx(c, b))] = v[i * M0::R0 + q];
         });
       }
     });
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) void k_col_pass_w(ColArgs a) {
     static_for<0, KST>([&](auto K) {
       constexpr int k = decltype(K)::value;
       const int i = tid + k * BS;
-      if ((STFULL || i < N * C) && i % C < ncol) wv_base[(i / C) * nx + i % C] = st[k];
+      if ((STFULL || i < N * C) && i % C < ncol) stg<C_::NT_COL>(wv_base + (i / C) * nx + i % C, st[k]);
     });
   }
 }
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_fft(float2* __restrict_
                                                                       const float2* __restrict__ twg) {
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
-  using L = LineLayout<N, C_::PADSH>;
+  using L = LineLayout<N, C_::PADSH, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   __shared__ float2 s_tw[N];
   __shared__ float2 s_buf[L::words(LPB)];
@@ -625,7 +626,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) void k_row_fft(float2* __restrict_
     });
   }
   __syncthreads();
-  LineFFT<N, T, DIR, P, false, L, NREG>::run(v, s_buf, line, s_tw, t, SyncBlock{});
+  LineFFT<N, T, DIR, P, false, L, NREG, C_::TWR>::run(v, s_buf, line, s_tw, t, SyncBlock{});
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -669,7 +670,7 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_fft(float2* __restrict__
     });
   }
   __syncthreads();
-  LineFFT<N, T, DIR, P, false, L, NREG>::run(v, s_buf, c, s_tw, t, SyncBlock{});
+  LineFFT<N, T, DIR, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncBlock{});
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -716,10 +717,19 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   using CF = Cfg<N>;
   if constexpr (CF::WL) {
     dim3 g((h.nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
-    if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true>), g, b, 0, st, a);
-    else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false>), g, b, 0, st, a);
-    else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_col_pass_w<N, false, false>), g, b, 0, st, a);
+    // detection planes get their own instantiation: the readout's registers
+    // would otherwise halve the occupancy of every other slice
+    if (h.detect) {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, true>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, true>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, true>), g, b, 0, st, a);
+    } else {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, false>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, false>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, false>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false>), g, b, 0, st, a);
+    }
   } else {
     dim3 g((h.nx + CF::C - 1) / CF::C, nwave), b(CF::TC * CF::C);
     if (init && exit_) hipLaunchKernelGGL((k_col_pass<N, true, true>), g, b, 0, st, a);
