@@ -14,7 +14,10 @@ Prints ONE JSON line.  Kernel durations come from HIP events recorded around
 every launch on the engine's stream inside the timed region; the roofline
 object reports the dominant kernel's algorithmic bytes (16 * n per wave: one
 read and one write of the complex64 wave) per average launch duration against
-MI355X's 8 TB/s HBM peak.  cpu_baseline times the CPU oracle (this project's
+MI355X's 8 TB/s HBM peak; its `traffic` is the HBM bytes per launch measured
+with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size)
+and committed in profiles/hbm_traffic_stem_scan_512.md, per wave times the
+waves per launch.  cpu_baseline times the CPU oracle (this project's
 restatement of the reference CPUMultislice, float32, one std::thread per
 core) on a bounded sample of the same workload.
 """
@@ -23,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -37,6 +41,20 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_stem_scan_512.md")
+
+
+def measured_traffic_per_wave(kernel):
+    """HBM bytes per wave per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (copy-calibrated), or None."""
+    try:
+        txt = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_wave\[" + re.escape(kernel) + r"\]\s*=\s*([0-9.]+)", txt)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(sp, dx, dy, var, npass, cores):
@@ -133,8 +151,11 @@ def main():
         nwave_avg = positions * npass * (len(sp.objslc) + (1 if dom == 1 else 0)) / max(cnt, 1)
         bytes_per_launch = 16.0 * n * nwave_avg
         achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+        tpw = measured_traffic_per_wave(kinds[dom]) if sp.nx == 512 and sp.ny == 512 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if tpw is None else round(tpw * nwave_avg),
+                "traffic_source": None if tpw is None else os.path.relpath(TRAFFIC_PROFILE, ROOT),
                 "kernel": kinds[dom], "bytes_per_launch": bytes_per_launch,
                 "avg_launch_ms": round(ms / cnt, 4)}
         total_kernel_ms = sum(v[0] for v in tm.values())
