@@ -45,6 +45,7 @@ struct Cfg {
   // memory phase (measured 1.35 -> 1.11 ms per 1024 waves on MI355X).
   static constexpr int WPE_COL = (N == 512) ? 8 : 1;
   static constexpr int WPE_ROW = 1;
+
   // nontemporal wave loads/stores in the row / column pass: the streamed
   // waves then leave the reused phase gratings and propagators in L2
   // (row pass 0.84 -> 0.78 ms per 1024 waves at N = 512)
@@ -52,9 +53,14 @@ struct Cfg {
   static constexpr bool NT_COL = N >= 256;
   // LDS position policy of line layouts (0 padded, 1 swizzled) and twiddle
   // powers by product tree (one table read per butterfly) instead of one
-  // table read per leg
+  // table read per leg; measured at N = 512 (2048 waves, interleaved runs):
+  // column pass 2.54 -> 2.21 ms with it, row pass 1.60 -> 1.63 ms, so the
+  // row pass keeps the table reads
   static constexpr int SW = 0;
   static constexpr bool TWR = true;
+  static constexpr bool TWR_ROW = false;
+  // fused tile I/O in the wave-local column kernel where the plan allows it
+  static constexpr bool FIO = true;
 };
 
 

@@ -432,9 +432,32 @@ __device__ __forceinline__ int lds_slot(int line, int b, int a0) {
 //   lds : LDS scratch for this line group (through Layout::idx(line, e))
 //   tw  : W_N^e = exp(-2 pi i e/N), e in [0, N) (forward twiddles)
 //   SYNC: callable issuing the barrier appropriate for the line group
+//   STOP: when >= 0, the transform returns right after writing the output of
+//         stage STOP to LDS (no barrier), so that the caller can finish the
+//         remaining stages with another thread mapping; -1 runs all stages.
+// LineFFT<..., S>::read_input() loads stage S's input layout from LDS (the
+// exchange read), for a caller that resumes a transform at stage S.
 // ---------------------------------------------------------------------------
-template <int N, int T, int DIR, class P, bool REV, class Layout, int NREG, bool TWR = false, int S = 0>
+template <int N, int T, int DIR, class P, bool REV, class Layout, int NREG, bool TWR = false, int STOP = -1,
+          int S = 0>
 struct LineFFT {
+  static __device__ __forceinline__ void read_input(float2 (&v)[NREG], const float2* lds, int line, int t) {
+    constexpr int R = radix_at<P, REV>(S);
+    constexpr int NBF = N / R;
+    constexpr int NB = cdiv(NBF, T);
+    static_for<0, NB>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const int b = t + i * T;
+      if ((NB * T == NBF) || b < NBF) {
+        const int a0 = Layout::idx(line, b);
+        static_for<0, R>([&](auto Q) {
+          constexpr int q = decltype(Q)::value;
+          v[i * R + q] = lds[lds_slot<Layout, NBF, q>(line, b, a0)];
+        });
+      }
+    });
+  }
+
   template <class Sync>
   static __device__ __forceinline__ void run(float2 (&v)[NREG], float2* lds, int line,
                                              const float2* tw, int t, Sync&& sync) {
@@ -443,6 +466,7 @@ struct LineFFT {
     constexpr int NBF = N / R;
     constexpr int NB = cdiv(NBF, T);
     constexpr bool LAST = (S == P::S - 1);
+    static_assert(STOP < P::S - 1, "STOP must leave a stage to the caller");
     static_for<0, NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
       const int b = t + i * T;
@@ -490,23 +514,12 @@ struct LineFFT {
           });
         }
       });
-      sync();
-      constexpr int R2 = radix_at<P, REV>(S + 1);
-      constexpr int NBF2 = N / R2;
-      constexpr int NB2 = cdiv(NBF2, T);
-      static_for<0, NB2>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        const int b = t + i * T;
-        if ((NB2 * T == NBF2) || b < NBF2) {
-          const int a0 = Layout::idx(line, b);
-          static_for<0, R2>([&](auto Q) {
-            constexpr int q = decltype(Q)::value;
-            v[i * R2 + q] = lds[lds_slot<Layout, NBF2, q>(line, b, a0)];
-          });
-        }
-      });
-      sync();
-      LineFFT<N, T, DIR, P, REV, Layout, NREG, TWR, S + 1>::run(v, lds, line, tw, t, sync);
+      if constexpr (S != STOP) {
+        sync();
+        LineFFT<N, T, DIR, P, REV, Layout, NREG, TWR, STOP, S + 1>::read_input(v, lds, line, t);
+        sync();
+        LineFFT<N, T, DIR, P, REV, Layout, NREG, TWR, STOP, S + 1>::run(v, lds, line, tw, t, sync);
+      }
     }
   }
 };

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   __syncthreads();
   // rows lie inside one wavefront when T divides 64: wave-level barriers suffice
   using RowSync = std::conditional_t<(64 % T == 0), SyncWave, SyncBlock>;
-  LineFFT<N, T, +1, P, false, L, NREG, C_::TWR>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, +1, P, false, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
       }
     });
   }
-  LineFFT<N, T, -1, P, true, L, NREG, C_::TWR>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, -1, P, true, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, s_tw, t, RowSync{});
   if (valid) {
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -370,47 +370,19 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
 }
 
 
-// ---------------------------------------------------------------------------
-// column pass, wave-local variant (same arguments and results as k_col_pass).
-// The tile of CW adjacent columns is staged through LDS with coalesced
-// row-segment loads and stores; each column is transformed inside one
-// wavefront (TW threads, TW divides 64), so the FFT exchanges need only
-// wave-level barriers.  Propagator, detector weights and phase plate are
-// read from column-major copies (pro_t, detw_t, cppl_t) so that the lanes of
-// a column access them contiguously.
-// grid: (ceil(nx/CW), nwave); block: TW*CW
-// ---------------------------------------------------------------------------
-template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
-__global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(ColArgs a) {
-  using P = typename C_::P;
-  constexpr int T = C_::TW, C = C_::CW, BS = T * C;
-  static_assert(64 % T == 0, "a column must lie inside one wavefront");
-  static_assert(BS % 64 == 0, "block must be whole wavefronts");
+// Tile of C adjacent columns x N rows staged through LDS (column c at
+// L::idx(c, e)); thread tid moves elements i = tid + k*BS, i.e. row i / C,
+// column i % C, so a wavefront covers 64/C whole 8*C-byte row segments.
+template <int N, class C_>
+struct ColTile {
+  static constexpr int T = C_::TW, C = C_::CW, BS = T * C;
+  static constexpr int KST = (N * C + BS - 1) / BS;
+  static constexpr bool STFULL = (N * C) % BS == 0;
   using L = ColLayout<N, C_::PADSH, C_::SW>;
-  constexpr int NREG = regs_needed<N, T, P>();
-  __shared__ float2 s_tw[N];
-  __shared__ float2 s_buf[L::words(C)];
-  __shared__ float s_red[(BS / 64) * kMaxDet];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  const int c = wv * (64 / T) + lane / T;  // column within the tile
-  const int t = lane % T;
-  for (int e = tid; e < N; e += BS) s_tw[e] = a.twg[e];
-  const long long w = blockIdx.y;
-  const int nx = a.nx;
-  const long long n = (long long)N * nx;
-  const int x0 = blockIdx.x * C;
-  const int ncol = (nx - x0 < C) ? (nx - x0) : C;
-  const int x = x0 + c;
-  const bool valid = c < ncol;
-  float2* wv_base = a.wave + w * a.wave_stride + x0;
-  // staging sweep: KST elements per thread, all loads issued before the LDS
-  // writes (columns past the grid edge read column x0 and are discarded)
-  constexpr int KST = (N * C + BS - 1) / BS;
-  constexpr bool STFULL = (N * C) % BS == 0;
-  {
-    const float2* src = INIT ? (a.src + w * a.src_stride + x0) : wv_base;
-    float2 st[KST];
+  // global -> registers (columns past the grid edge read column 0 of the
+  // tile and are discarded)
+  static __device__ __forceinline__ void load(float2 (&st)[KST], const float2* src, int nx, int ncol,
+                                              int tid) {
     static_for<0, KST>([&](auto K) {
       constexpr int k = decltype(K)::value;
       const int i = tid + k * BS;
@@ -419,16 +391,52 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
         st[k] = ldg<C_::NT_COL>(src + e * nx + (cc < ncol ? cc : 0));
       }
     });
+  }
+  static __device__ __forceinline__ void to_lds(const float2 (&st)[KST], float2* s_buf, int tid) {
     static_for<0, KST>([&](auto K) {
       constexpr int k = decltype(K)::value;
       const int i = tid + k * BS;
       if (STFULL || i < N * C) s_buf[L::idx(i % C, i / C)] = st[k];
     });
   }
-  __syncthreads();
+  static __device__ __forceinline__ void from_lds(float2 (&st)[KST], const float2* s_buf, int tid) {
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if (STFULL || i < N * C) st[k] = s_buf[L::idx(i % C, i / C)];
+    });
+  }
+  static __device__ __forceinline__ void store(const float2 (&st)[KST], float2* dst, int nx, int ncol,
+                                               int tid) {
+    static_for<0, KST>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      const int i = tid + k * BS;
+      if ((STFULL || i < N * C) && i % C < ncol) stg<C_::NT_COL>(dst + (i / C) * nx + i % C, st[k]);
+    });
+  }
+};
+
+// The per-column work of the column pass on a tile already in LDS, done by
+// the wavefront lanes of column c (thread t of T): forward FFT along y (or,
+// for INIT, the phase plate on the probe), * propagator, readout, inverse
+// FFT along y (unless EXIT); the result is left in the column's LDS region.
+// Only wave-level barriers inside, except for the readout's block reduction.
+// FIN: the forward transform's first stage was done by the caller (its
+// output is in LDS), so the body resumes at stage 1.  FOUT: the inverse
+// transform stops after writing its second-to-last stage to LDS; the caller
+// does the last stage.
+template <int N, bool INIT, bool EXIT, bool DET, class C_, bool FIN = false, bool FOUT = false>
+__device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* s_buf, const float2* s_tw,
+                                         float* s_red, int c, int t, int x, bool valid) {
+  using P = typename C_::P;
+  constexpr int T = C_::TW;
+  using L = ColLayout<N, C_::PADSH, C_::SW>;
+  constexpr int NREG = regs_needed<N, T, P>();
   using M0 = IOMap<N, T, P, false>;
   using M1 = IOMap<N, T, P, true>;
-  const long long colb = (long long)(valid ? x : x0) * N;  // column-major offset
+  const int nx = a.nx;
+  const long long n = (long long)N * nx;
+  const long long colb = (long long)(valid ? x : x - c) * N;  // column-major offset
   float2 v[NREG];
   if constexpr (INIT) {
     float dx = 0.f, dy = 0.f, fz = 0.f;
@@ -460,18 +468,24 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
       }
     });
   } else {
-    static_for<0, M0::NB>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      const int b = t + i * T;
-      if (M0::FULL || b < M0::NBF) {
-        static_for<0, M0::R0>([&](auto Q) {
-          constexpr int q = decltype(Q)::value;
-          v[i * M0::R0 + q] = s_buf[lds_slot<L, M0::NBF, q>(c, b, L::idx(c, b))];
-        });
-      }
-    });
-    SyncWave{}();
-    LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    if constexpr (FIN) {
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, -1, 1>::read_input(v, s_buf, c, t);
+      SyncWave{}();
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, -1, 1>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    } else {
+      static_for<0, M0::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M0::FULL || b < M0::NBF) {
+          static_for<0, M0::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            v[i * M0::R0 + q] = s_buf[lds_slot<L, M0::NBF, q>(c, b, L::idx(c, b))];
+          });
+        }
+      });
+      SyncWave{}();
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    }
     const float2* pcol = a.pro_t + colb;
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -562,6 +576,9 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
         });
       }
     });
+  } else if constexpr (FOUT) {
+    SyncWave{}();
+    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR, P::S - 2>::run(v, s_buf, c, s_tw, t, SyncWave{});
   } else {
     SyncWave{}();
     LineFFT<N, T, +1, P, true, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
@@ -577,19 +594,71 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
       }
     });
   }
+}
+
+// ---------------------------------------------------------------------------
+// column pass, wave-local form (N <= 768): the block stages a tile of CW
+// adjacent columns through LDS with coalesced row-segment loads and stores;
+// each column is transformed inside one wavefront (TW threads, TW divides 64),
+// so the FFT exchanges need only wave-level barriers.  Propagator, detector
+// weights and phase plate are read from column-major copies (pro_t, detw_t,
+// cppl_t) so that the lanes of a column access them contiguously.
+// grid: (ceil(nx/CW), nwave); block: TW*CW
+// ---------------------------------------------------------------------------
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
+__global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(ColArgs a) {
+  constexpr int T = C_::TW, C = C_::CW, BS = T * C;
+  static_assert(64 % T == 0, "a column must lie inside one wavefront");
+  static_assert(BS % 64 == 0, "block must be whole wavefronts");
+  using L = ColLayout<N, C_::PADSH, C_::SW>;
+  using TL = ColTile<N, C_>;
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_buf[L::words(C)];
+  __shared__ float s_red[(BS / 64) * kMaxDet];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int c = wv * (64 / T) + lane / T;  // column within the tile
+  const int t = lane % T;
+  for (int e = tid; e < N; e += BS) s_tw[e] = a.twg[e];
+  const long long w = blockIdx.y;
+  const int nx = a.nx;
+  const int x0 = blockIdx.x * C;
+  const int ncol = (nx - x0 < C) ? (nx - x0) : C;
+  float2* wv_base = a.wave + w * a.wave_stride + x0;
+  // Fused tile I/O: when a column's first-stage butterflies are exactly the
+  // T threads per column, the staging sweep already leaves in thread tid the
+  // inputs of butterfly tid / C of column tid % C (rows b + k*N/R0), so the
+  // first forward stage runs on the staged registers and writes straight into
+  // the exchange layout, and the last inverse stage (the same radix, reversed
+  // plan) produces the rows that the store sweep writes.  This removes the
+  // LDS round trips of staging in and out.
+  using P = typename C_::P;
+  constexpr int NREG = regs_needed<N, T, P>();
+  constexpr bool FIO = C_::FIO && P::S >= 2 && N / radix_at<P, false>(0) == T && TL::KST == NREG;
+  constexpr bool FIN = FIO && !INIT, FOUT = FIO && !EXIT;
+  const int cs = tid % C, bs = tid / C;  // sweep mapping: column, first-stage butterfly
+  {
+    float2 st[TL::KST];
+    TL::load(st, INIT ? (a.src + w * a.src_stride + x0) : wv_base, nx, ncol, tid);
+    if constexpr (FIN) {
+      // twiddles from the global table: s_tw is complete only after the barrier
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, 0>::run(st, s_buf, cs, a.twg, bs, SyncWave{});
+    } else {
+      TL::to_lds(st, s_buf, tid);
+    }
+  }
   __syncthreads();
-  {  // store the tile: all LDS reads first, then the (guarded) global stores
-    float2 st[KST];
-    static_for<0, KST>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      const int i = tid + k * BS;
-      if (STFULL || i < N * C) st[k] = s_buf[L::idx(i % C, i / C)];
-    });
-    static_for<0, KST>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      const int i = tid + k * BS;
-      if ((STFULL || i < N * C) && i % C < ncol) stg<C_::NT_COL>(wv_base + (i / C) * nx + i % C, st[k]);
-    });
+  col_body<N, INIT, EXIT, DET, C_, FIN, FOUT>(a, w, s_buf, s_tw, s_red, c, t, x0 + c, c < ncol);
+  __syncthreads();
+  {
+    float2 st[TL::KST];
+    if constexpr (FOUT) {
+      LineFFT<N, T, +1, P, true, L, NREG, C_::TWR, -1, P::S - 1>::read_input(st, s_buf, cs, bs);
+      LineFFT<N, T, +1, P, true, L, NREG, C_::TWR, -1, P::S - 1>::run(st, s_buf, cs, s_tw, bs, SyncWave{});
+    } else {
+      TL::from_lds(st, s_buf, tid);
+    }
+    TL::store(st, wv_base, nx, ncol, tid);
   }
 }
 
