@@ -84,6 +84,8 @@ JMS_API int SetCurrentGPU(int idev);                                /* :207 */
 JMS_API int GetGPUStats(int idev, int* iCMajor, int* iCMinor, int* iMaxThread,
                         int64_t* memtotal, int64_t* memfree);       /* :210 */
 JMS_API int GetUnscrambleHash(unsigned int* phash);                 /* :220 */
+JMS_API float GetImageAveragingWeight(int whichcode, int iThread);  /* :223 */
+JMS_API float GetWaveAveragingWeight(int whichcode, int iThread);   /* :226 */
 
 /* setup interface */
 JMS_API int PhaseGratingSetup(int whichcode, int nx, int ny, int nslc, int nvarmax,
@@ -100,6 +102,8 @@ JMS_API int SetSlicePhaseGratings(int whichcode, int islc, int nvar, float* pgr)
 JMS_API int SetPropagatorData(int whichcode, int ipro, float* pro); /* :258 */
 JMS_API int SetDetectorData(int whichcode, int idet, float* det, int msklen, int* msk); /* :261 */
 JMS_API int SetGPUPgrLoading(int npgrload);                         /* :266 */
+JMS_API int ResetImageAveraging(int whichcode, int iThread);        /* :269 */
+JMS_API int ResetWaveAveraging(int whichcode, int iThread);         /* :272 */
 
 /* calculation interface */
 JMS_API int CalculateProbeWaveFourier(CJProbeParams* prm, float* wav);   /* :280 */
@@ -113,6 +117,8 @@ JMS_API int OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int 
                                int ibtx, int ibty, int bcppl);           /* :326 */
 JMS_API int GPUMultislice(int islc0, int accmode, float weight);         /* :339 */
 JMS_API int GetResult(int whichcode, int whichresult, float* dst, int iThread); /* :346 */
+JMS_API int GetAvgResult(int whichcode, int whichresult, float* dst, float* wgt,
+                         int iThread);                                  /* :354 */
 JMS_API int ClearDetMem_d(void);                                         /* :363 */
 
 /* batched STEM scan (no counterpart in the reference; additive).  For each

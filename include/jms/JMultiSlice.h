@@ -8,9 +8,11 @@
 // Differences from the reference, by design:
 //  - The CPU code path (_JMS_CODE_CPU: CPUMultislice and the host multislice
 //    buffers) is not part of this library; requests for it return an error.
-//  - Real-space readout channels (_JMS_DETECT_IMAGE, _WAVEREAL and their
-//    averages), plasmon Monte-Carlo, diffraction de-scan and sub-frame phase
-//    gratings are not implemented; setups that need them fail with an error.
+//  - Plasmon Monte-Carlo, diffraction de-scan and sub-frame phase gratings
+//    are not implemented; setups that need them fail with an error.
+//  - The detection channels (integrated, image, diffraction, real- and
+//    Fourier-space waves and their averages) are filled by the single-wave
+//    path (GPUMultislice); the batched scan fills the integrated channel.
 //  - Probe aberrations other than none are not implemented
 //    (CalculateProbeWaveFourier returns an error if a coefficient is set).
 //  - Additive batched entry GPUMultisliceScan runs many positions and passes
@@ -130,6 +132,14 @@ class CJMultiSlice {
   int SetIncomingWaveGPU(fcmplx* wav, bool bTranspose = false);
   int GPUMultislice(int islc0, int accmode, float weight = 1.0f);
   int GetResult(int whichcode, int whichresult, float* dst, int iThread = 0);
+  // averaging channels (JMultiSlice.h:903-925)
+  int GetAvgResult(int whichcode, int whichresult, float* dst, float& weight, int iThread = 0);
+  float GetImageAveragingWeight(int whichcode, int iThread = 0);
+  float GetWaveAveragingWeight(int whichcode, int iThread = 0);
+  int ResetImageAveraging(int whichcode, int iThread = 0);
+  int ResetWaveAveraging(int whichcode, int iThread = 0);
+  // intensities of the averaged (elastic) waves (JMultiSlice.h:927-936)
+  int ReadoutDetAvg(int whichcode, int whichresult, float* dst, float& weight, int iThread = 0);
 
   // batched STEM scan (additive entry): for every position p and pass r,
   // the equivalent of OffsetIncomingWave(GPU, dx[p], dy[p], dz[p]) +

@@ -23,8 +23,15 @@ typedef struct jms_hip_engine jms_hip_engine;
 
 /* detection channel flags (JMultiSlice.h:141-150) */
 #define JMS_HIP_DETECT_INTEGRATED 1
+#define JMS_HIP_DETECT_IMAGE 2
 #define JMS_HIP_DETECT_DIFFRACTION 4
+#define JMS_HIP_DETECT_WAVEREAL 8
 #define JMS_HIP_DETECT_WAVEFOURIER 16
+#define JMS_HIP_DETECT_IMAGE_AVG 32
+#define JMS_HIP_DETECT_DIFFR_AVG 64
+#define JMS_HIP_DETECT_WAVER_AVG 128
+#define JMS_HIP_DETECT_WAVEF_AVG 256
+#define JMS_HIP_DETECT_ALL 511
 
 /* engine lifetime ---------------------------------------------------------- */
 int jms_hip_create(jms_hip_engine** out);
@@ -91,11 +98,29 @@ int jms_hip_wave_set(jms_hip_engine* e, const float* wav);
  * channels when accumulate == 0 -- GPUMultislice :4513 */
 int jms_hip_multislice(jms_hip_engine* e, int islc0, const int* var, float weight, int accumulate);
 /* copy a result channel of the single-wave path: which = JMS_HIP_DETECT_*;
- * INTEGRATED: ndetslc*ndet floats; DIFFRACTION: ndetslc*nx*ny floats;
- * WAVEFOURIER: 2*ndetslc*nx*ny floats -- GetResult :2406 */
+ * INTEGRATED: ndetslc*ndet floats; DIFFRACTION, IMAGE: ndetslc*nx*ny floats;
+ * WAVEFOURIER, WAVEREAL: 2*ndetslc*nx*ny floats -- GetResult :2406.  IMAGE
+ * and WAVEREAL hold |psi(r)|^2 and psi(r) of the unscaled inverse transform
+ * of each detection plane (ReadoutImgDet_d :4369). */
 int jms_hip_get_result(jms_hip_engine* e, int which, float* dst);
 /* copy the current wave (2*nx*ny floats) */
 int jms_hip_wave_get(jms_hip_engine* e, float* dst);
+/* averaging channels of the single-wave path: they accumulate over runs
+ * (jms_hip_multislice does not clear them) until reset; which =
+ * JMS_HIP_DETECT_IMAGE_AVG / DIFFR_AVG (ndetslc*nx*ny floats, weight = summed
+ * run weights of the image group) or WAVER_AVG / WAVEF_AVG (2*ndetslc*nx*ny
+ * floats, weight of the wave group) -- GetAvgResult :2489 */
+int jms_hip_get_avg_result(jms_hip_engine* e, int which, float* dst, float* weight);
+/* summed run weights of the image and wave averaging groups -- :2567-2590 */
+int jms_hip_get_averaging_weights(jms_hip_engine* e, float* img_weight, float* wave_weight);
+/* group 0: image group (IMAGE_AVG, DIFFR_AVG), group 1: wave group (WAVER_AVG,
+ * WAVEF_AVG); clears the channels and the weight -- ResetImageAveraging :2592,
+ * ResetWaveAveraging :2629 */
+int jms_hip_reset_averaging(jms_hip_engine* e, int group);
+/* intensities of the averaged (elastic) waves: INTEGRATED (ndetslc*ndet
+ * floats, from the averaged Fourier-space wave), DIFFRACTION or IMAGE
+ * (ndetslc*nx*ny floats); weight = wave-group weight -- ReadoutDetAvg :2667 */
+int jms_hip_readout_det_avg(jms_hip_engine* e, int which, float* dst, float* weight);
 
 /* utilities ---------------------------------------------------------------- */
 /* in-place 2-D FFT of nbatch host arrays (dir < 0: forward exp(-i), dir > 0:

@@ -48,6 +48,8 @@ _SIGS = [
     ("SetCurrentGPU", c_int, [c_int]),
     ("GetGPUStats", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("GetUnscrambleHash", c_int, [c_void_p]),
+    ("GetImageAveragingWeight", c_float, [c_int, c_int]),
+    ("GetWaveAveragingWeight", c_float, [c_int, c_int]),
     ("PhaseGratingSetup", c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p]),
     ("ObjectSliceSetup", c_int, [c_int, c_void_p]),
     ("PropagatorSetup", c_int, [c_int, c_int, c_void_p]),
@@ -60,6 +62,8 @@ _SIGS = [
     ("SetPropagatorData", c_int, [c_int, c_int, c_void_p]),
     ("SetDetectorData", c_int, [c_int, c_int, c_void_p, c_int, c_void_p]),
     ("SetGPUPgrLoading", c_int, [c_int]),
+    ("ResetImageAveraging", c_int, [c_int, c_int]),
+    ("ResetWaveAveraging", c_int, [c_int, c_int]),
     ("CalculateProbeWaveFourier", c_int, [c_void_p, c_void_p]),
     ("SetIncomingWaveGPU", c_int, [c_void_p, c_int]),
     ("CalculatePropagator", c_int, [c_float, c_float, c_float, c_void_p, c_int]),
@@ -68,6 +72,7 @@ _SIGS = [
     ("OffsetIncomingWave", c_int, [c_int, c_float, c_float, c_float, c_int, c_int, c_int, c_int]),
     ("GPUMultislice", c_int, [c_int, c_int, c_float]),
     ("GetResult", c_int, [c_int, c_int, c_void_p, c_int]),
+    ("GetAvgResult", c_int, [c_int, c_int, c_void_p, c_void_p, c_int]),
     ("ClearDetMem_d", c_int, []),
     ("GPUMultisliceScan", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("GPUMultisliceScanVar", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
@@ -101,6 +106,10 @@ _SIGS = [
     ("jms_hip_wave_get", c_int, [c_void_p, c_void_p]),
     ("jms_hip_multislice", c_int, [c_void_p, c_int, c_void_p, c_float, c_int]),
     ("jms_hip_get_result", c_int, [c_void_p, c_int, c_void_p]),
+    ("jms_hip_get_avg_result", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    ("jms_hip_get_averaging_weights", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("jms_hip_reset_averaging", c_int, [c_void_p, c_int]),
+    ("jms_hip_readout_det_avg", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("jms_hip_fft2d", c_int, [c_int, c_int, c_int, c_void_p, c_int]),
     ("jms_hip_set_timing", c_int, [c_void_p, c_int]),
     ("jms_hip_get_timing", c_int, [c_void_p, c_int, c_void_p, c_void_p]),

@@ -564,10 +564,8 @@ int CJMultiSlice::DetectorSetup(int whichcode, std::vector<int> slc_det, int nde
 int CJMultiSlice::DetectorChannels(int whichcode, int ndetint, int imagedet) {
   m_imagedet = imagedet;
   m_ndet = std::max(0, ndetint);
-  const int supported = _JMS_DETECT_INTEGRATED | _JMS_DETECT_DIFFRACTION | _JMS_DETECT_WAVEFOURIER;
-  if (imagedet & ~supported) {
-    std::cerr << "Error: (DetectorSetup): requested detection channels are not implemented "
-                 "(real-space and averaging channels)." << std::endl;
+  if (imagedet & ~JMS_HIP_DETECT_ALL) {
+    std::cerr << "Error: (DetectorSetup): unknown detection channel flags." << std::endl;
     return 102;
   }
   m_d_det_int.assign((size_t)m_ndet * m_ndetslc, 0.f);
@@ -766,10 +764,57 @@ int CJMultiSlice::GetResult(int whichcode, int whichresult, float* dst, int iThr
     std::memcpy(dst, m_d_det_int.data(), sizeof(float) * m_d_det_int.size());
     return 0;
   }
-  if (whichresult == _JMS_DETECT_DIFFRACTION || whichresult == _JMS_DETECT_WAVEFOURIER) {
+  if (whichresult == _JMS_DETECT_DIFFRACTION || whichresult == _JMS_DETECT_WAVEFOURIER ||
+      whichresult == _JMS_DETECT_IMAGE || whichresult == _JMS_DETECT_WAVEREAL) {
     return jms_hip_get_result(m_eng, whichresult, dst) ? 102 : 0;
   }
   return 1;
+}
+
+int CJMultiSlice::GetAvgResult(int whichcode, int whichresult, float* dst, float& weight, int iThread) {
+  (void)iThread;
+  weight = 0.f;
+  if (!dst) return 0;
+  if (cpu_code(whichcode)) return reject_cpu("GetAvgResult");
+  if (!gpu_code(whichcode) || m_ndetslc <= 0 || !m_eng) return 0;
+  return jms_hip_get_avg_result(m_eng, whichresult, dst, &weight) ? 102 : 0;
+}
+
+float CJMultiSlice::GetImageAveragingWeight(int whichcode, int iThread) {
+  (void)iThread;
+  float wi = 0.f;
+  if (gpu_code(whichcode) && m_eng) jms_hip_get_averaging_weights(m_eng, &wi, nullptr);
+  return wi;
+}
+
+float CJMultiSlice::GetWaveAveragingWeight(int whichcode, int iThread) {
+  (void)iThread;
+  float ww = 0.f;
+  if (gpu_code(whichcode) && m_eng) jms_hip_get_averaging_weights(m_eng, nullptr, &ww);
+  return ww;
+}
+
+int CJMultiSlice::ResetImageAveraging(int whichcode, int iThread) {
+  (void)iThread;
+  if (cpu_code(whichcode)) return reject_cpu("ResetImageAveraging");
+  if (!gpu_code(whichcode) || !m_eng) return 0;
+  return jms_hip_reset_averaging(m_eng, 0) ? 101 : 0;
+}
+
+int CJMultiSlice::ResetWaveAveraging(int whichcode, int iThread) {
+  (void)iThread;
+  if (cpu_code(whichcode)) return reject_cpu("ResetWaveAveraging");
+  if (!gpu_code(whichcode) || !m_eng) return 0;
+  return jms_hip_reset_averaging(m_eng, 1) ? 103 : 0;
+}
+
+int CJMultiSlice::ReadoutDetAvg(int whichcode, int whichresult, float* dst, float& weight, int iThread) {
+  (void)iThread;
+  weight = 0.f;
+  if (!dst) return 0;
+  if (cpu_code(whichcode)) return reject_cpu("ReadoutDetAvg");
+  if (!gpu_code(whichcode) || m_ndetslc <= 0 || !m_eng) return 0;
+  return jms_hip_readout_det_avg(m_eng, whichresult, dst, &weight) ? 121 : 0;
 }
 
 int CJMultiSlice::GPUMultisliceScanVar(int npos, const float* dx, const float* dy, const float* dz, int npass,

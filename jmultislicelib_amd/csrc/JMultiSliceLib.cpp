@@ -54,6 +54,12 @@ int GetGPUStats(int idev, int* iCMajor, int* iCMinor, int* iMaxThread, int64_t* 
   return rc;
 }
 int GetUnscrambleHash(unsigned int* phash) { return jms_obj().GetUnscrambleHash(phash); }
+float GetImageAveragingWeight(int whichcode, int iThread) {
+  return jms_obj().GetImageAveragingWeight(whichcode, iThread);
+}
+float GetWaveAveragingWeight(int whichcode, int iThread) {
+  return jms_obj().GetWaveAveragingWeight(whichcode, iThread);
+}
 
 int PhaseGratingSetup(int whichcode, int nx, int ny, int nslc, int nvarmax, int* nslcvar) {
   return jms_obj().PhaseGratingSetup(whichcode, nx, ny, nslc, nvarmax, nslcvar);
@@ -81,6 +87,8 @@ int SetDetectorData(int whichcode, int idet, float* det, int msklen, int* msk) {
   return jms_obj().SetDetectorData(whichcode, idet, det, msklen, msk);
 }
 int SetGPUPgrLoading(int npgrload) { return jms_obj().SetGPUPgrLoading(npgrload); }
+int ResetImageAveraging(int whichcode, int iThread) { return jms_obj().ResetImageAveraging(whichcode, iThread); }
+int ResetWaveAveraging(int whichcode, int iThread) { return jms_obj().ResetWaveAveraging(whichcode, iThread); }
 
 int CalculateProbeWaveFourier(CJProbeParams* prm, float* wav) {
   return jms_obj().CalculateProbeWaveFourier(prm, (fcmplx*)wav);
@@ -103,6 +111,13 @@ int OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int iThread,
   return jms_obj().OffsetIncomingWave(whichcode, dx, dy, dz, iThread, ibtx, ibty, bcppl != 0);
 }
 int GPUMultislice(int islc0, int accmode, float weight) { return jms_obj().GPUMultislice(islc0, accmode, weight); }
+int GetAvgResult(int whichcode, int whichresult, float* dst, float* wgt, int iThread) {
+  float w = 0.f;
+  const int rc = jms_obj().GetAvgResult(whichcode, whichresult, dst, w, iThread);
+  if (wgt) *wgt = w;
+  return rc;
+}
+
 int GetResult(int whichcode, int whichresult, float* dst, int iThread) {
   return jms_obj().GetResult(whichcode, whichresult, dst, iThread);
 }

@@ -87,8 +87,12 @@ struct jms_hip_engine {
   // batch working buffers
   int bcap_req = 0;
   DevBuf waves, partial, cur_var, cur_ppar, cur_res;
-  // single-wave path
-  DevBuf wav, var1, res1, w1, dif1, wff1;
+  // single-wave path: wave, variant sequence, results and channels
+  DevBuf wav, var1, res1, w1, dif1, wff1, img1, wfr1;
+  // averaging channels (JMultiSlice.cpp:2489-2660): accumulated over runs until
+  // reset, with the summed run weights of the image and wave groups
+  DevBuf dif_avg, wff_avg, img_avg, wfr_avg;
+  float w_imgavg = 0.f, w_wavavg = 0.f;
   // graph cache: key = (nwave, islc0)
   bool use_graph = false;
   std::map<std::pair<int, int>, hipGraphExec_t> graphs;
@@ -188,10 +192,24 @@ struct TimedLaunch {
 // ints per wave) describe the waves; src/src_stride the INIT source.
 // Readout partial sums go to e->partial, combined into `res` (npos results)
 // when npass > 0.
+// per-wave readout channels of a batch (any may be null; cwgt is the
+// per-wave weight, the channels hold ndetslc planes of n items per wave)
+struct BatchChans {
+  float* dif = nullptr;
+  float* dif_avg = nullptr;
+  float* img = nullptr;
+  float* img_avg = nullptr;
+  void* wff = nullptr;
+  void* wff_avg = nullptr;
+  void* wfr = nullptr;
+  void* wfr_avg = nullptr;
+  const float* cwgt = nullptr;
+  bool real_space() const { return img || img_avg || wfr || wfr_avg; }
+};
+
 int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long long src_stride,
                   const void* ppar, const int* var, int npos, int npass, const float* weights,
-                  float* res, int accumulate, const float* cwgt, float* dif, void* wff,
-                  bool timed) {
+                  float* res, int accumulate, const BatchChans& ch, bool timed) {
   const int S = e->nobjslc;
   const int ntc = e->ntiles_col();
   const long long part_w = (long long)e->ndetslc * e->ndet * ntc;
@@ -209,10 +227,19 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   ca.partial = e->partial.as<float>();
   ca.part_wstride = part_w;
   ca.part_dstride = ntc;
-  ca.dif = dif;
-  ca.wff = wff;
-  ca.cwgt = cwgt;
+  ca.dif = ch.dif;
+  ca.wff = ch.wff;
+  ca.dif_avg = ch.dif_avg;
+  ca.wff_avg = ch.wff_avg;
+  ca.cwgt = ch.cwgt;
   ca.chan_wstride = (long long)e->ndetslc * e->n;
+  jms::RowChanHost rc;
+  rc.img = ch.img;
+  rc.img_avg = ch.img_avg;
+  rc.wfr = ch.wfr;
+  rc.wfr_avg = ch.wfr_avg;
+  rc.cwgt = ch.cwgt;
+  rc.chan_wstride = ca.chan_wstride;
   ca.twy = e->twy.p;
   hipError_t err;
   auto plane_setup = [&](int plane) {
@@ -237,10 +264,14 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   for (int j = islc0; j < S; ++j) {
     const int js = e->objslc[j] % e->nscslc;
     {
+      // real-space readout of plane j inside the row pass (ReadoutImgDet_d)
+      const int id = e->det_objslc[j];
+      const int mode = (id >= 0 && ch.real_space()) ? 1 : 0;
+      rc.chan_off = id >= 0 ? (long long)id * e->n : 0;
       TimedLaunch tl(e, 0, timed);
       err = jms::launch_row_pass(e->nx, e->ny, nwave, e->waves.as<float2>(), e->n,
                                  e->pgr.as<float2>(), e->slcoffset[js], (long long)e->npgx * e->npgy,
-                                 var, S, j, e->twx.as<float2>(), e->st);
+                                 var, S, j, e->twx.as<float2>(), rc, mode, e->st);
       if (err != hipSuccess) return e->fail(131, "row pass", err);
     }
     ca.pro = e->pro.as<float2>() + (long long)e->proidx[js] * e->n;
@@ -258,6 +289,28 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
                               res, accumulate, e->st);
     if (err != hipSuccess) return e->fail(133, "combine", err);
   }
+  return 0;
+}
+
+// exit-plane real-space readout (GPUMultislice :4662-4666): the column IFFT
+// in place, then the inverse row transforms with the readout; the batch
+// slots are left in a mixed state and must be treated as scratch.
+int enqueue_exit_real_readout(jms_hip_engine* e, int nwave, const BatchChans& ch) {
+  const int id = e->det_objslc[e->nobjslc];
+  if (id < 0 || !ch.real_space()) return 0;
+  jms::RowChanHost rc;
+  rc.img = ch.img;
+  rc.img_avg = ch.img_avg;
+  rc.wfr = ch.wfr;
+  rc.wfr_avg = ch.wfr_avg;
+  rc.cwgt = ch.cwgt;
+  rc.chan_wstride = (long long)e->ndetslc * e->n;
+  rc.chan_off = (long long)id * e->n;
+  hipError_t err = jms::launch_col_fft(e->nx, e->ny, nwave, e->waves.as<float2>(), +1, e->twy.as<float2>(), e->st);
+  if (err == hipSuccess)
+    err = jms::launch_row_pass(e->nx, e->ny, nwave, e->waves.as<float2>(), e->n, e->pgr.as<float2>(), 0, 0,
+                               nullptr, 0, 0, e->twx.as<float2>(), rc, 2, e->st);
+  if (err != hipSuccess) return e->fail(134, "exit-plane real-space readout", err);
   return 0;
 }
 
@@ -316,7 +369,8 @@ void jms_hip_destroy(jms_hip_engine* e) {
                     &e->cppl, &e->cppl_t,
                     &e->s_ppar, &e->s_weights, &e->s_var, &e->s_res, &e->waves, &e->partial,
                     &e->cur_var, &e->cur_ppar, &e->cur_res, &e->wav, &e->var1, &e->res1,
-                    &e->w1, &e->dif1, &e->wff1};
+                    &e->w1, &e->dif1, &e->wff1, &e->img1, &e->wfr1, &e->dif_avg, &e->wff_avg,
+                    &e->img_avg, &e->wfr_avg};
   for (DevBuf* b : bufs) b->release();
   if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
@@ -427,8 +481,7 @@ int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objsl
   if (!e) return 1;
   if (!det_objslc || ndetslc < 1 || ndet < 0) return e->fail(2, "invalid detection setup");
   if (ndet > jms::kMaxDetPerSweep) return e->fail(3, "too many integrating detectors (max 8)");
-  if (flags & ~(JMS_HIP_DETECT_INTEGRATED | JMS_HIP_DETECT_DIFFRACTION | JMS_HIP_DETECT_WAVEFOURIER))
-    return e->fail(4, "unsupported detection channel (real-space channels are not implemented)");
+  if (flags & ~JMS_HIP_DETECT_ALL) return e->fail(4, "unknown detection channel flag");
   for (int i = 0; i <= e->nobjslc; ++i)
     if (det_objslc[i] >= ndetslc) return e->fail(5, "detection index out of range");
   (void)hipSetDevice(e->device);
@@ -437,6 +490,23 @@ int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objsl
   e->ndet = ndet;
   e->detflags = flags;
   e->det_objslc.assign(det_objslc, det_objslc + e->nobjslc + 1);
+  // averaging channels start empty (DetectorSetup, JMultiSlice.cpp:784-800)
+  {
+    const size_t nchan = (size_t)ndetslc * e->n;
+    struct { DevBuf* b; int flag; size_t nb; } avg[] = {
+        {&e->dif_avg, JMS_HIP_DETECT_DIFFR_AVG, sizeof(float) * nchan},
+        {&e->img_avg, JMS_HIP_DETECT_IMAGE_AVG, sizeof(float) * nchan},
+        {&e->wff_avg, JMS_HIP_DETECT_WAVEF_AVG, 2 * sizeof(float) * nchan},
+        {&e->wfr_avg, JMS_HIP_DETECT_WAVER_AVG, 2 * sizeof(float) * nchan}};
+    for (auto& a : avg) {
+      if (!(flags & a.flag)) { a.b->release(); continue; }
+      hipError_t err = a.b->ensure(a.nb);
+      if (err == hipSuccess) err = hipMemset(a.b->p, 0, a.nb);
+      if (err != hipSuccess) return e->fail(104, "allocate averaging channels", err);
+    }
+    e->w_imgavg = 0.f;
+    e->w_wavavg = 0.f;
+  }
   if (ndet > 0) {
     hipError_t err = e->detw.ensure(sizeof(float) * (size_t)ndet * e->n);
     if (err == hipSuccess) err = hipMemset(e->detw.p, 0, sizeof(float) * (size_t)ndet * e->n);
@@ -608,8 +678,7 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
         err = hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) return e->fail(121, "begin capture", err);
         rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, e->cur_ppar.p, e->cur_var.as<int>(), np,
-                           npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, nullptr,
-                           nullptr, nullptr, false);
+                           npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, BatchChans(), false);
         hipError_t err2 = hipStreamEndCapture(e->st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         if (err2 != hipSuccess) return e->fail(122, "end capture", err2);
@@ -625,7 +694,7 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
       if (err != hipSuccess) return e->fail(124, "graph launch", err);
     } else {
       rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, ppar, var, np, npass, e->s_weights.as<float>(), res,
-                         0, nullptr, nullptr, nullptr, e->timing);
+                         0, BatchChans(), e->timing);
       if (rc) return rc;
     }
   }
@@ -708,28 +777,48 @@ int jms_hip_multislice(jms_hip_engine* e, int islc0, const int* var, float weigh
   hipError_t err = e->res1.ensure(sizeof(float) * per);
   if (err == hipSuccess) err = e->w1.ensure(sizeof(float));
   if (err == hipSuccess && S > 0) err = e->var1.ensure(sizeof(int) * S);
-  const bool wdif = (e->detflags & JMS_HIP_DETECT_DIFFRACTION) != 0;
-  const bool wwff = (e->detflags & JMS_HIP_DETECT_WAVEFOURIER) != 0;
-  if (err == hipSuccess && wdif) err = e->dif1.ensure(sizeof(float) * nchan);
-  if (err == hipSuccess && wwff) err = e->wff1.ensure(sizeof(float) * 2 * nchan);
+  const int fl = e->detflags;
+  // per-run channels (cleared unless accumulating) and their buffers
+  struct { DevBuf* b; int flag; size_t nb; } run[] = {
+      {&e->dif1, JMS_HIP_DETECT_DIFFRACTION, sizeof(float) * nchan},
+      {&e->img1, JMS_HIP_DETECT_IMAGE, sizeof(float) * nchan},
+      {&e->wff1, JMS_HIP_DETECT_WAVEFOURIER, 2 * sizeof(float) * nchan},
+      {&e->wfr1, JMS_HIP_DETECT_WAVEREAL, 2 * sizeof(float) * nchan}};
+  for (auto& r : run)
+    if (err == hipSuccess && (fl & r.flag)) err = r.b->ensure(r.nb);
   if (err != hipSuccess) return e->fail(110, "allocate single-wave buffers", err);
-  if (!accumulate) {  // ClearDetMem_d (JMultiSlice.cpp:4054-4108)
+  if (!accumulate) {  // ClearDetMem_d (JMultiSlice.cpp:4054-4108); the averaging channels stay
     err = hipMemsetAsync(e->res1.p, 0, sizeof(float) * per, e->st);
-    if (err == hipSuccess && wdif) err = hipMemsetAsync(e->dif1.p, 0, sizeof(float) * nchan, e->st);
-    if (err == hipSuccess && wwff) err = hipMemsetAsync(e->wff1.p, 0, sizeof(float) * 2 * nchan, e->st);
+    for (auto& r : run)
+      if (err == hipSuccess && (fl & r.flag)) err = hipMemsetAsync(r.b->p, 0, r.nb, e->st);
     if (err != hipSuccess) return e->fail(111, "clear detector channels", err);
   }
   err = hipMemcpyAsync(e->w1.p, &weight, sizeof(float), hipMemcpyHostToDevice, e->st);
   if (err == hipSuccess && S > 0) err = hipMemcpyAsync(e->var1.p, var, sizeof(int) * S, hipMemcpyHostToDevice, e->st);
   if (err != hipSuccess) return e->fail(112, "stage single wave", err);
+  BatchChans ch;
+  ch.cwgt = e->w1.as<float>();
+  if (fl & JMS_HIP_DETECT_DIFFRACTION) ch.dif = e->dif1.as<float>();
+  if (fl & JMS_HIP_DETECT_IMAGE) ch.img = e->img1.as<float>();
+  if (fl & JMS_HIP_DETECT_WAVEFOURIER) ch.wff = e->wff1.p;
+  if (fl & JMS_HIP_DETECT_WAVEREAL) ch.wfr = e->wfr1.p;
+  if (fl & JMS_HIP_DETECT_DIFFR_AVG) ch.dif_avg = e->dif_avg.as<float>();
+  if (fl & JMS_HIP_DETECT_IMAGE_AVG) ch.img_avg = e->img_avg.as<float>();
+  if (fl & JMS_HIP_DETECT_WAVEF_AVG) ch.wff_avg = e->wff_avg.p;
+  if (fl & JMS_HIP_DETECT_WAVER_AVG) ch.wfr_avg = e->wfr_avg.p;
   rc = enqueue_batch(e, 1, islc0, e->wav.p, e->n, nullptr, e->var1.as<int>(), 1, 1, e->w1.as<float>(),
-                     e->res1.as<float>(), 1, e->w1.as<float>(), wdif ? e->dif1.as<float>() : nullptr,
-                     wwff ? e->wff1.p : nullptr, e->timing);
+                     e->res1.as<float>(), 1, ch, e->timing);
   if (rc) return rc;
   // the run leaves psi(k) of the exit plane in the batch slot; keep it as the wave
   err = hipMemcpyAsync(e->wav.p, e->waves.p, sizeof(float) * 2 * (size_t)e->n, hipMemcpyDeviceToDevice, e->st);
-  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
   if (err != hipSuccess) return e->fail(113, "multislice run", err);
+  rc = enqueue_exit_real_readout(e, 1, ch);
+  if (rc) return rc;
+  err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(113, "multislice run", err);
+  // averaging weights count the runs that fed the averaging channels (:4668-4673)
+  if (fl & (JMS_HIP_DETECT_DIFFR_AVG | JMS_HIP_DETECT_IMAGE_AVG)) e->w_imgavg += weight;
+  if (fl & (JMS_HIP_DETECT_WAVEF_AVG | JMS_HIP_DETECT_WAVER_AVG)) e->w_wavavg += weight;
   return 0;
 }
 
@@ -748,10 +837,102 @@ int jms_hip_get_result(jms_hip_engine* e, int which, float* dst) {
   } else if (which == JMS_HIP_DETECT_WAVEFOURIER) {
     if (!e->wff1.p) return e->fail(4, "wave channel not active");
     err = hipMemcpy(dst, e->wff1.p, sizeof(float) * 2 * nchan, hipMemcpyDeviceToHost);
+  } else if (which == JMS_HIP_DETECT_IMAGE) {
+    if (!e->img1.p) return e->fail(6, "image channel not active");
+    err = hipMemcpy(dst, e->img1.p, sizeof(float) * nchan, hipMemcpyDeviceToHost);
+  } else if (which == JMS_HIP_DETECT_WAVEREAL) {
+    if (!e->wfr1.p) return e->fail(7, "real-space wave channel not active");
+    err = hipMemcpy(dst, e->wfr1.p, sizeof(float) * 2 * nchan, hipMemcpyDeviceToHost);
   } else {
     return e->fail(5, "unsupported result channel");
   }
   if (err != hipSuccess) return e->fail(102, "read result", err);
+  return 0;
+}
+
+int jms_hip_get_avg_result(jms_hip_engine* e, int which, float* dst, float* weight) {
+  if (!e || !dst) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(101, "synchronize", err);
+  const size_t nchan = (size_t)e->ndetslc * e->n;
+  const DevBuf* b = nullptr;
+  size_t nb = 0;
+  float w = 0.f;
+  switch (which) {  // GetAvgResult GPU branch (JMultiSlice.cpp:2534-2560)
+    case JMS_HIP_DETECT_IMAGE_AVG: b = &e->img_avg; nb = sizeof(float) * nchan; w = e->w_imgavg; break;
+    case JMS_HIP_DETECT_DIFFR_AVG: b = &e->dif_avg; nb = sizeof(float) * nchan; w = e->w_imgavg; break;
+    case JMS_HIP_DETECT_WAVER_AVG: b = &e->wfr_avg; nb = 2 * sizeof(float) * nchan; w = e->w_wavavg; break;
+    case JMS_HIP_DETECT_WAVEF_AVG: b = &e->wff_avg; nb = 2 * sizeof(float) * nchan; w = e->w_wavavg; break;
+    default: return e->fail(5, "unsupported averaging channel");
+  }
+  if (!b->p) return e->fail(6, "averaging channel not active");
+  err = hipMemcpy(dst, b->p, nb, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(102, "read averaging channel", err);
+  if (weight) *weight = w;
+  return 0;
+}
+
+int jms_hip_get_averaging_weights(jms_hip_engine* e, float* img_weight, float* wave_weight) {
+  if (!e) return 1;
+  if (img_weight) *img_weight = e->w_imgavg;
+  if (wave_weight) *wave_weight = e->w_wavavg;
+  return 0;
+}
+
+int jms_hip_reset_averaging(jms_hip_engine* e, int group) {
+  if (!e) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  const size_t nchan = (size_t)e->ndetslc * e->n;
+  if (group == 0) {  // ResetImageAveraging (JMultiSlice.cpp:2592-2627)
+    e->w_imgavg = 0.f;
+    if (err == hipSuccess && e->img_avg.p) err = hipMemset(e->img_avg.p, 0, sizeof(float) * nchan);
+    if (err == hipSuccess && e->dif_avg.p) err = hipMemset(e->dif_avg.p, 0, sizeof(float) * nchan);
+  } else if (group == 1) {  // ResetWaveAveraging (:2629-2664)
+    e->w_wavavg = 0.f;
+    if (err == hipSuccess && e->wfr_avg.p) err = hipMemset(e->wfr_avg.p, 0, 2 * sizeof(float) * nchan);
+    if (err == hipSuccess && e->wff_avg.p) err = hipMemset(e->wff_avg.p, 0, 2 * sizeof(float) * nchan);
+  } else {
+    return e->fail(2, "unknown averaging group");
+  }
+  if (err != hipSuccess) return e->fail(101, "reset averaging channels", err);
+  return 0;
+}
+
+int jms_hip_readout_det_avg(jms_hip_engine* e, int which, float* dst, float* weight) {
+  if (!e || !dst) return 1;
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipStreamSynchronize(e->st);
+  if (err != hipSuccess) return e->fail(101, "synchronize", err);
+  const size_t n = (size_t)e->n, nchan = (size_t)e->ndetslc * n;
+  if (weight) *weight = e->w_wavavg;
+  // intensities of the averaged waves (ReadoutDetAvg GPU branch, JMultiSlice.cpp:2806-2890)
+  const DevBuf* src = nullptr;
+  if (which == JMS_HIP_DETECT_IMAGE) src = &e->wfr_avg;
+  else if (which == JMS_HIP_DETECT_INTEGRATED || which == JMS_HIP_DETECT_DIFFRACTION) src = &e->wff_avg;
+  else return e->fail(5, "unsupported result channel");
+  if (!src->p) return e->fail(6, "the averaged wave of that space is not collected");
+  std::vector<float> wv(2 * nchan);
+  err = hipMemcpy(wv.data(), src->p, sizeof(float) * 2 * nchan, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(102, "read averaged wave", err);
+  if (which != JMS_HIP_DETECT_INTEGRATED) {
+    for (size_t i = 0; i < nchan; ++i) dst[i] = wv[2 * i] * wv[2 * i] + wv[2 * i + 1] * wv[2 * i + 1];
+    return 0;
+  }
+  if (e->ndet <= 0) return 0;
+  std::vector<float> det((size_t)e->ndet * n);
+  err = hipMemcpy(det.data(), e->detw.p, sizeof(float) * det.size(), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(103, "read detector functions", err);
+  for (int p = 0; p < e->ndetslc; ++p) {
+    const float* w = wv.data() + 2 * n * p;
+    for (int d = 0; d < e->ndet; ++d) {
+      const float* dw = det.data() + n * d;
+      double acc = 0.0;
+      for (size_t i = 0; i < n; ++i) acc += (double)(w[2 * i] * w[2 * i] + w[2 * i + 1] * w[2 * i + 1]) * dw[i];
+      dst[(size_t)p * e->ndet + d] = (float)acc;
+    }
+  }
   return 0;
 }
 

@@ -24,6 +24,8 @@ struct ColArgsHost {
   long long part_off = 0;        // offset of this detection plane
   float* dif = nullptr;          // diffraction channel |psi|^2 (or nullptr)
   void* wff = nullptr;           // Fourier-space wave channel (float2, or nullptr)
+  float* dif_avg = nullptr;      // averaged diffraction channel (or nullptr)
+  void* wff_avg = nullptr;       // averaged Fourier-space wave channel (or nullptr)
   const float* cwgt = nullptr;   // per batch entry weight of the channels
   long long chan_wstride = 0;    // per batch entry stride of the channels (items)
   long long chan_off = 0;        // offset of this detection plane in the channels
@@ -34,15 +36,33 @@ struct ColArgsHost {
   const float* cppl_t = nullptr; // constant phase plate or nullptr
 };
 
+// real-space readout channels of one detection plane for the row pass
+// (float, or float2 for the wave channels; any may be nullptr)
+struct RowChanHost {
+  float* img = nullptr;
+  float* img_avg = nullptr;
+  void* wfr = nullptr;
+  void* wfr_avg = nullptr;
+  const float* cwgt = nullptr;   // per batch entry weight
+  long long chan_wstride = 0;    // per batch entry stride (items)
+  long long chan_off = 0;        // offset of this detection plane (items)
+};
+
 constexpr int kMaxDetPerSweep = 8;
 
 bool size_supported(int n);
 int row_tiles(int nx, int ny);
 int col_tiles(int ny, int nx);
 
+// mode 0: slice; 1: slice with the real-space readout of `rc` first;
+// 2: inverse x-transform and readout only (exit plane, after a column IFFT)
 hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
                            const float2* pgr, long long slc_off, long long npg, const int* var_tab,
-                           int var_ld, int jslc, const float2* twx, hipStream_t st);
+                           int var_ld, int jslc, const float2* twx, const RowChanHost& rc, int mode,
+                           hipStream_t st);
+// column transforms only (all columns of nbatch grids)
+hipError_t launch_col_fft(int nx, int ny, int nbatch, float2* data, int dir, const float2* twy,
+                          hipStream_t st);
 hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColArgsHost& a,
                            hipStream_t st);
 hipError_t launch_fft2d(int nx, int ny, int nbatch, float2* data, int dir, const float2* twx,

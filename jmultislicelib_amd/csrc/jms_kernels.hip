@@ -14,7 +14,7 @@ template <int N>
 hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
                              const float2* pgr, long long slc_off, long long npg,
                              const int* var_tab, int var_ld, int jslc, const float2* twx,
-                             hipStream_t st);
+                             const RowChanHost& rc, int mode, hipStream_t st);
 template <int N>
 hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost& h,
                              hipStream_t st);
@@ -119,12 +119,13 @@ int col_tiles(int ny, int nx) {
 
 hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
                            const float2* pgr, long long slc_off, long long npg, const int* var_tab,
-                           int var_ld, int jslc, const float2* twx, hipStream_t st) {
+                           int var_ld, int jslc, const float2* twx, const RowChanHost& rc, int mode,
+                           hipStream_t st) {
   switch (nx) {
 #define JMS_CASE(N)                                                                         \
   case N:                                                                                   \
     return launch_row_pass_n<N>(ny, nwave, wave, wave_stride, pgr, slc_off, npg, var_tab, \
-                                var_ld, jslc, twx, st);
+                                var_ld, jslc, twx, rc, mode, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;
@@ -135,6 +136,17 @@ hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColAr
                            hipStream_t st) {
   switch (ny) {
 #define JMS_CASE(N) case N: return launch_col_pass_n<N>(nwave, init, exit_, a, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_col_fft(int nx, int ny, int nbatch, float2* data, int dir, const float2* twy,
+                          hipStream_t st) {
+  const long long stride = (long long)nx * ny;
+  switch (ny) {
+#define JMS_CASE(N) case N: return launch_col_fft_n<N>(nx, nbatch, data, stride, dir, twy, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;

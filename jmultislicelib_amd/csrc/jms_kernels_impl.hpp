@@ -60,15 +60,34 @@ __device__ __forceinline__ int fidx(int i, int n) {
   return ((i + h) % n) - h;
 }
 
+// real-space readout channels of one detection plane (per wave: weight
+// cwgt[w], channel stride chan_wstride items, plane offset chan_off items)
+struct RowChan {
+  float* img;
+  float* img_avg;
+  float2* wfr;
+  float2* wfr_avg;
+  const float* cwgt;
+  long long chan_wstride;
+  long long chan_off;
+};
+
 // ---------------------------------------------------------------------------
 // row pass: wave[w] rows <- FFT_x( t[jslc, var] * IFFT_x(rows) )
+//   MODE 0: the slice;
+//   MODE 1: the slice, with the real-space readout of the plane before it
+//           (ReadoutImgDet_d, JMultiSlice.cpp:4369-4465) between IFFT_x and
+//           the grating multiply, where the wave is IFFT2 of the plane's
+//           Fourier-space wave (the column pass did IFFT_y);
+//   MODE 2: IFFT_x and the readout only (exit plane after a column IFFT);
+//           the wave is not written back.
 // grid: (ceil(ny/LPB), nwave); block: T*LPB
 // ---------------------------------------------------------------------------
-template <int N, class C_ = Cfg<N>>
+template <int N, int MODE = 0, class C_ = Cfg<N>>
 __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu(C_::WPE_ROW))) void k_row_pass(
     float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
     long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
-    const float2* __restrict__ twg) {
+    const float2* __restrict__ twg, RowChan rc) {
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH, C_::SW>;
@@ -82,7 +101,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   const int row = blockIdx.x * LPB + line;
   const bool valid = row < ny;
   float2* wrow = wave + w * wave_stride + (long long)row * N;
-  const int var = var_tab[w * var_ld + jslc];
+  const int var = MODE == 2 ? 0 : var_tab[w * var_ld + jslc];
   const float2* grow = pgr + slc_off + (long long)var * npg + (long long)row * N;
   using M0 = IOMap<N, T, P, false>;
   using M1 = IOMap<N, T, P, true>;
@@ -103,6 +122,29 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   // rows lie inside one wavefront when T divides 64: wave-level barriers suffice
   using RowSync = std::conditional_t<(64 % T == 0), SyncWave, SyncBlock>;
   LineFFT<N, T, +1, P, false, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, s_tw, t, RowSync{});
+  if constexpr (MODE >= 1) {
+    if (valid) {  // weight * |psi(r)|^2 and weight * psi(r) into the channels of this plane
+      const float wg = rc.cwgt[w];
+      const long long cb = w * rc.chan_wstride + rc.chan_off + (long long)row * N;
+      static_for<0, M1::NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const int b = t + i * T;
+        if (M1::FULL || b < M1::NBF) {
+          static_for<0, M1::R0>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const long long idx = cb + b + q * M1::NBF;
+            const float2 z = v[i * M1::R0 + q];
+            const float pz = z.x * z.x + z.y * z.y;
+            if (rc.img) rc.img[idx] += wg * pz;
+            if (rc.img_avg) rc.img_avg[idx] += wg * pz;
+            if (rc.wfr) { float2 o = rc.wfr[idx]; o.x += z.x * wg; o.y += z.y * wg; rc.wfr[idx] = o; }
+            if (rc.wfr_avg) { float2 o = rc.wfr_avg[idx]; o.x += z.x * wg; o.y += z.y * wg; rc.wfr_avg[idx] = o; }
+          });
+        }
+      });
+    }
+  }
+  if constexpr (MODE == 2) return;
   if (valid) {
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -170,6 +212,8 @@ __device__ __forceinline__ void block_reduce_det(float (&acc)[kMaxDet], int ndet
 // grid: (ceil(nx/C), nwave); block: TC*C
 // ---------------------------------------------------------------------------
 struct ColArgs {
+  float* dif_avg;           // averaged diffraction channel (or nullptr)
+  float2* wff_avg;          // averaged Fourier-space wave channel (or nullptr)
   float2* wave;
   int nx;
   long long wave_stride;
@@ -316,7 +360,7 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
     block_reduce_det(acc, a.ndet, (float*)s_buf,
                      a.partial + w * a.part_wstride + a.part_off + blockIdx.x, a.part_dstride);
     __syncthreads();
-    if (valid && (a.dif || a.wff)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
+    if (valid && (a.dif || a.wff || a.dif_avg || a.wff_avg)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
       const float wg = a.cwgt[w];
       const long long cb = w * a.chan_wstride + a.chan_off;
       static_for<0, M1::NB>([&](auto I) {
@@ -327,13 +371,11 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
             constexpr int q = decltype(Q)::value;
             const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
             const float2 z = v[i * M1::R0 + q];
-            if (a.dif) a.dif[idx] += wg * (z.x * z.x + z.y * z.y);
-            if (a.wff) {
-              float2 o = a.wff[idx];
-              o.x += z.x * wg;
-              o.y += z.y * wg;
-              a.wff[idx] = o;
-            }
+            const float pz = z.x * z.x + z.y * z.y;
+            if (a.dif) a.dif[idx] += wg * pz;
+            if (a.dif_avg) a.dif_avg[idx] += wg * pz;
+            if (a.wff) { float2 o = a.wff[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff[idx] = o; }
+            if (a.wff_avg) { float2 o = a.wff_avg[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff_avg[idx] = o; }
           });
         }
       });
@@ -541,7 +583,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
     }
     block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + blockIdx.x,
                      a.part_dstride);
-    if (valid && (a.dif || a.wff)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
+    if (valid && (a.dif || a.wff || a.dif_avg || a.wff_avg)) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
       const float wg = a.cwgt[w];
       const long long cb = w * a.chan_wstride + a.chan_off;
       static_for<0, M1::NB>([&](auto I) {
@@ -552,13 +594,11 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
             constexpr int q = decltype(Q)::value;
             const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
             const float2 z = v[i * M1::R0 + q];
-            if (a.dif) a.dif[idx] += wg * (z.x * z.x + z.y * z.y);
-            if (a.wff) {
-              float2 o = a.wff[idx];
-              o.x += z.x * wg;
-              o.y += z.y * wg;
-              a.wff[idx] = o;
-            }
+            const float pz = z.x * z.x + z.y * z.y;
+            if (a.dif) a.dif[idx] += wg * pz;
+            if (a.dif_avg) a.dif_avg[idx] += wg * pz;
+            if (a.wff) { float2 o = a.wff[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff[idx] = o; }
+            if (a.wff_avg) { float2 o = a.wff_avg[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff_avg[idx] = o; }
           });
         }
       });
@@ -761,10 +801,21 @@ template <int N>
 hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
                              const float2* pgr, long long slc_off, long long npg,
                              const int* var_tab, int var_ld, int jslc, const float2* twx,
-                             hipStream_t st) {
+                             const RowChanHost& h, int mode, hipStream_t st) {
   dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nwave), b(Cfg<N>::T * Cfg<N>::LPB);
-  hipLaunchKernelGGL(k_row_pass<N>, g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
-                     var_tab, var_ld, jslc, twx);
+  RowChan rc;
+  rc.img = h.img; rc.img_avg = h.img_avg;
+  rc.wfr = (float2*)h.wfr; rc.wfr_avg = (float2*)h.wfr_avg;
+  rc.cwgt = h.cwgt; rc.chan_wstride = h.chan_wstride; rc.chan_off = h.chan_off;
+  if (mode == 1)
+    hipLaunchKernelGGL((k_row_pass<N, 1>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
+                       var_tab, var_ld, jslc, twx, rc);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_row_pass<N, 2>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
+                       var_tab, var_ld, jslc, twx, rc);
+  else
+    hipLaunchKernelGGL((k_row_pass<N, 0>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
+                       var_tab, var_ld, jslc, twx, rc);
   return hipGetLastError();
 }
 
@@ -778,6 +829,7 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   a.detect = h.detect; a.ndet = h.ndet; a.detw = h.detw; a.partial = h.partial;
   a.part_wstride = h.part_wstride; a.part_dstride = h.part_dstride; a.part_off = h.part_off;
   a.dif = h.dif; a.wff = (float2*)h.wff; a.cwgt = h.cwgt; a.chan_wstride = h.chan_wstride;
+  a.dif_avg = h.dif_avg; a.wff_avg = (float2*)h.wff_avg;
   a.chan_off = h.chan_off;
   a.twg = (const float2*)h.twy;
   a.pro_t = (const float2*)h.pro_t;

@@ -9,7 +9,7 @@
 namespace jms {
 template hipError_t launch_row_pass_n<JMS_N>(int, int, float2*, long long, const float2*, long long,
                                              long long, const int*, int, int, const float2*,
-                                             hipStream_t);
+                                             const RowChanHost&, int, hipStream_t);
 template hipError_t launch_col_pass_n<JMS_N>(int, bool, bool, const ColArgsHost&, hipStream_t);
 template hipError_t launch_row_fft_n<JMS_N>(int, int, float2*, long long, int, const float2*,
                                             hipStream_t);

@@ -20,8 +20,14 @@ _JMS_CODE_GPU = 2
 _JMS_ACCMODE_NONE = 0
 _JMS_ACCMODE_INTEGRATE = 1
 _JMS_DETECT_INTEGRATED = 1
+_JMS_DETECT_IMAGE = 2
 _JMS_DETECT_DIFFRACTION = 4
+_JMS_DETECT_WAVEREAL = 8
 _JMS_DETECT_WAVEFOURIER = 16
+_JMS_DETECT_IMAGE_AVG = 32
+_JMS_DETECT_DIFFR_AVG = 64
+_JMS_DETECT_WAVER_AVG = 128
+_JMS_DETECT_WAVEF_AVG = 256
 
 
 class JMSError(RuntimeError):
@@ -175,19 +181,40 @@ class JMultiSlice:
     def GPUMultislice(self, islc0: int, accmode: int, weight: float = 1.0) -> int:
         return self.L.GPUMultislice(islc0, accmode, weight)
 
-    def GetResult(self, whichcode, whichresult) -> np.ndarray:
+    def _chan_buffer(self, which):
         nx, ny = self.GetGridSize()
         nds = self.GetDetetionSliceNum()
-        if whichresult == _JMS_DETECT_INTEGRATED:
-            out = np.zeros((nds, self.GetDetNum()), np.float32)
-        elif whichresult == _JMS_DETECT_DIFFRACTION:
-            out = np.zeros((nds, ny, nx), np.float32)
-        elif whichresult == _JMS_DETECT_WAVEFOURIER:
-            out = np.zeros((nds, ny, nx), np.complex64)
-        else:
-            raise ValueError("unsupported result channel")
+        if which == _JMS_DETECT_INTEGRATED:
+            return np.zeros((nds, self.GetDetNum()), np.float32)
+        if which in (_JMS_DETECT_DIFFRACTION, _JMS_DETECT_IMAGE, _JMS_DETECT_IMAGE_AVG, _JMS_DETECT_DIFFR_AVG):
+            return np.zeros((nds, ny, nx), np.float32)
+        if which in (_JMS_DETECT_WAVEFOURIER, _JMS_DETECT_WAVEREAL, _JMS_DETECT_WAVER_AVG, _JMS_DETECT_WAVEF_AVG):
+            return np.zeros((nds, ny, nx), np.complex64)
+        raise ValueError("unsupported result channel")
+
+    def GetResult(self, whichcode, whichresult) -> np.ndarray:
+        out = self._chan_buffer(whichresult)
         _chk("GetResult", self.L.GetResult(whichcode, whichresult, ptr(out), 0))
         return out
+
+    def GetAvgResult(self, whichcode, whichresult):
+        """(averaged channel, weight) -- GetAvgResult (JMultiSliceLib.h:354)."""
+        out = self._chan_buffer(whichresult)
+        w = C.c_float(0.0)
+        _chk("GetAvgResult", self.L.GetAvgResult(whichcode, whichresult, ptr(out), C.byref(w), 0))
+        return out, w.value
+
+    def GetImageAveragingWeight(self, whichcode) -> float:
+        return self.L.GetImageAveragingWeight(whichcode, 0)
+
+    def GetWaveAveragingWeight(self, whichcode) -> float:
+        return self.L.GetWaveAveragingWeight(whichcode, 0)
+
+    def ResetImageAveraging(self, whichcode) -> int:
+        return self.L.ResetImageAveraging(whichcode, 0)
+
+    def ResetWaveAveraging(self, whichcode) -> int:
+        return self.L.ResetWaveAveraging(whichcode, 0)
 
     def GetRandomVariantSequence(self, nobjslc: int) -> np.ndarray:
         v = np.zeros(nobjslc, np.int32)
@@ -316,16 +343,40 @@ class Engine:
         v = _i32(var) if var is not None and len(var) else np.zeros(1, np.int32)
         self._c("jms_hip_multislice", islc0, ptr(v), weight, int(accumulate))
 
-    def result(self, which=_JMS_DETECT_INTEGRATED) -> np.ndarray:
+    def _buffer(self, which):
         sp = self.sp
         if which == _JMS_DETECT_INTEGRATED:
-            out = np.zeros((sp.ndetslc, len(sp.detectors)), np.float32)
-        elif which == _JMS_DETECT_DIFFRACTION:
-            out = np.zeros((sp.ndetslc, sp.ny, sp.nx), np.float32)
-        else:
-            out = np.zeros((sp.ndetslc, sp.ny, sp.nx), np.complex64)
+            return np.zeros((sp.ndetslc, len(sp.detectors)), np.float32)
+        if which in (_JMS_DETECT_DIFFRACTION, _JMS_DETECT_IMAGE, _JMS_DETECT_IMAGE_AVG, _JMS_DETECT_DIFFR_AVG):
+            return np.zeros((sp.ndetslc, sp.ny, sp.nx), np.float32)
+        return np.zeros((sp.ndetslc, sp.ny, sp.nx), np.complex64)
+
+    def result(self, which=_JMS_DETECT_INTEGRATED) -> np.ndarray:
+        out = self._buffer(which)
         self._c("jms_hip_get_result", which, ptr(out))
         return out
+
+    def avg_result(self, which):
+        """(averaged channel, weight) of the single-wave path."""
+        out = self._buffer(which)
+        w = C.c_float(0.0)
+        self._c("jms_hip_get_avg_result", which, ptr(out), C.byref(w))
+        return out, w.value
+
+    def averaging_weights(self):
+        wi, ww = C.c_float(0.0), C.c_float(0.0)
+        self._c("jms_hip_get_averaging_weights", C.byref(wi), C.byref(ww))
+        return wi.value, ww.value
+
+    def reset_averaging(self, group: int):
+        self._c("jms_hip_reset_averaging", group)
+
+    def readout_det_avg(self, which):
+        """(intensities of the averaged waves, wave-group weight)."""
+        out = self._buffer(which)
+        w = C.c_float(0.0)
+        self._c("jms_hip_readout_det_avg", which, ptr(out), C.byref(w))
+        return out, w.value
 
 
 def fft2d(a: np.ndarray, direction: int) -> np.ndarray:

@@ -812,15 +812,58 @@ struct OrcSetup {
   int imagedet;            // flags: 1 integrated, 16 wave Fourier
 };
 
-static const int kDetInt = 1, kDetWFF = 16;
+// detection flags (JMultiSlice.h:141-150); the _AVG channels accumulate the
+// same quantities as their plain counterparts into separate buffers, so the
+// oracle computes only the plain ones
+static const int kDetInt = 1, kDetImg = 2, kDetDif = 4, kDetWFR = 8, kDetWFF = 16;
+
+// optional per-plane output channels (ndetslc planes of n items each):
+// dif, img: intensities; wfr, wff: complex interleaved
+template <typename T>
+struct OrcChan {
+  T* dif = nullptr;
+  T* img = nullptr;
+  T* wfr = nullptr;
+};
+
+// ReadoutImgDet_h (JMultiSlice.cpp:3473-3538): the real-space wave at a
+// detection plane (the unscaled inverse transform of the Fourier-space wave)
+template <typename T>
+static void readout_img(const OrcSetup& S, const std::complex<T>* wav, int iplane, float weight,
+                        const OrcChan<T>& ch) {
+  const int n = S.nx * S.ny;
+  const int idetslc = S.det_objslc[iplane];
+  if (idetslc < 0) return;
+  if ((S.imagedet & kDetImg) && ch.img) {
+    T* o = ch.img + (size_t)n * idetslc;
+    for (int i = 0; i < n; ++i) {  // GetDataPow then out += weight * img
+      const T p = wav[i].real() * wav[i].real() + wav[i].imag() * wav[i].imag();
+      o[i] += (T)weight * p;
+    }
+  }
+  if ((S.imagedet & kDetWFR) && ch.wfr) {
+    T* o = ch.wfr + (size_t)2 * n * idetslc;
+    for (int i = 0; i < n; ++i) {
+      o[2 * i] += wav[i].real() * (T)weight;
+      o[2 * i + 1] += wav[i].imag() * (T)weight;
+    }
+  }
+}
 
 template <typename T>
 static void readout_dif(const OrcSetup& S, const std::complex<T>* wav, int iplane, float weight,
                         float* out_int, T* out_wff, std::vector<float>& difbuf,
-                        std::vector<T>& difbuf_d) {
+                        std::vector<T>& difbuf_d, const OrcChan<T>& ch = OrcChan<T>()) {
   const int n = S.nx * S.ny;
   const int idetslc = S.det_objslc[iplane];
   if (idetslc < 0) return;
+  if ((S.imagedet & kDetDif) && ch.dif) {  // ReadoutDifDet_h (:3369-3470): out += weight * |psi|^2
+    T* o = ch.dif + (size_t)n * idetslc;
+    for (int i = 0; i < n; ++i) {
+      const T p = wav[i].real() * wav[i].real() + wav[i].imag() * wav[i].imag();
+      o[i] += (T)weight * p;
+    }
+  }
   if (S.ndet > 0 && (S.imagedet & kDetInt)) {
     if (sizeof(T) == 4) {
       for (int i = 0; i < n; ++i) {  // JFFTMKLcore::GetDataPow (:690-706)
@@ -857,16 +900,19 @@ static void readout_dif(const OrcSetup& S, const std::complex<T>* wav, int iplan
 template <typename T>
 static int multislice_run(const OrcSetup& S, FFT2D<T>& fft, std::complex<T>* wav, int islc0,
                           const int* var, float weight, float* out_int, T* out_wff,
-                          std::vector<float>& difbuf, std::vector<T>& difbuf_d) {
+                          std::vector<float>& difbuf, std::vector<T>& difbuf_d,
+                          const OrcChan<T>& ch = OrcChan<T>()) {
   const int n = S.nx * S.ny;
+  const bool bimg = (S.imagedet & (kDetImg | kDetWFR)) != 0;
   int islc = islc0;
   if (islc < 0) islc = 0;
   if (islc >= S.nobjslc) islc = S.nobjslc;
   if (islc < S.nobjslc) {
     for (int jslc = islc; jslc < S.nobjslc; ++jslc) {
       const bool bdetect = S.det_objslc[jslc] >= 0;
-      if (bdetect) readout_dif<T>(S, wav, jslc, weight, out_int, out_wff, difbuf, difbuf_d);
+      if (bdetect) readout_dif<T>(S, wav, jslc, weight, out_int, out_wff, difbuf, difbuf_d, ch);
       fft.run(wav, +1);  // IFT
+      if (bdetect && bimg) readout_img<T>(S, wav, jslc, weight, ch);
       const int js = S.objslc[jslc] % S.nscslc;
       const std::complex<float>* pg =
           (const std::complex<float>*)(S.pgr) + S.slcoffset[js] + (int64_t)var[jslc] * n;
@@ -879,7 +925,11 @@ static int multislice_run(const OrcSetup& S, FFT2D<T>& fft, std::complex<T>* wav
         wav[i] = cmul(wav[i], std::complex<T>((T)pr[i].real(), (T)pr[i].imag()));
     }
   }
-  readout_dif<T>(S, wav, S.nobjslc, weight, out_int, out_wff, difbuf, difbuf_d);
+  readout_dif<T>(S, wav, S.nobjslc, weight, out_int, out_wff, difbuf, difbuf_d, ch);
+  if (bimg) {  // exit-plane real-space readout (JMultiSlice.cpp:3700-3712)
+    fft.run(wav, +1);
+    readout_img<T>(S, wav, S.nobjslc, weight, ch);
+  }
   return 0;
 }
 
@@ -925,6 +975,39 @@ ORC_API int orc_multislice(int prec, int nx, int ny, int nscslc, const int* nvar
   std::memcpy((void*)w.data(), wave_in, sizeof(float) * 2 * n);
   std::vector<float> dd(1);
   return multislice_run<float>(S, f, w.data(), islc0, var, weight, out_int, (float*)out_wff, difbuf, dd);
+}
+
+// orc_multislice with the intensity and wave channels: out_dif and out_img
+// (n*ndetslc items), out_wfr and out_wff (2*n*ndetslc items), each NULL or
+// accumulated with `weight`; float for prec 32, double for prec 64.
+ORC_API int orc_multislice_ch(int prec, int nx, int ny, int nscslc, const int* nvarslc,
+                              const long long* slcoffset, const float* pgr, int npro,
+                              const int* proidx, const float* pro, int nobjslc, const int* objslc,
+                              const int* det_objslc, int ndetslc, int ndet, const float* det,
+                              const int* msk, const int* msklen, int imagedet, const float* wave_in,
+                              const int* var, int islc0, float weight, float* out_int, void* out_dif,
+                              void* out_img, void* out_wfr, void* out_wff) {
+  if (nx < 1 || ny < 1 || !wave_in || !det_objslc) return 1;
+  OrcSetup S = make_setup(nx, ny, nscslc, nvarslc, slcoffset, pgr, npro, proidx, pro, nobjslc,
+                          objslc, det_objslc, ndetslc, ndet, det, msk, msklen, imagedet);
+  const int n = nx * ny;
+  std::vector<float> difbuf(n);
+  if (prec == 64) {
+    FFT2D<double> f; f.init(nx, ny);
+    std::vector<std::complex<double>> w(n);
+    for (int i = 0; i < n; ++i) w[i] = std::complex<double>(wave_in[2 * i], wave_in[2 * i + 1]);
+    std::vector<double> dd(n);
+    OrcChan<double> ch;
+    ch.dif = (double*)out_dif; ch.img = (double*)out_img; ch.wfr = (double*)out_wfr;
+    return multislice_run<double>(S, f, w.data(), islc0, var, weight, out_int, (double*)out_wff, difbuf, dd, ch);
+  }
+  FFT2D<float> f; f.init(nx, ny);
+  std::vector<std::complex<float>> w(n);
+  std::memcpy((void*)w.data(), wave_in, sizeof(float) * 2 * n);
+  std::vector<float> dd(1);
+  OrcChan<float> ch;
+  ch.dif = (float*)out_dif; ch.img = (float*)out_img; ch.wfr = (float*)out_wfr;
+  return multislice_run<float>(S, f, w.data(), islc0, var, weight, out_int, (float*)out_wff, difbuf, dd, ch);
 }
 
 // A STEM scan the way the reference bench drives the library

@@ -40,6 +40,8 @@ _SIGS = {
     "orc_offset_wave": (i_, [i_, i_, f_, f_, f_, p_, p_, f_, f_, f_, i_, i_, p_]),
     "orc_multislice": (i_, [i_, i_, i_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_, p_, i_, p_,
                             p_, i_, f_, p_, p_]),
+    "orc_multislice_ch": (i_, [i_, i_, i_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_, p_, i_, p_,
+                               p_, i_, f_, p_, p_, p_, p_, p_]),
     "orc_stem_scan": (i_, [i_, i_, i_, i_, f_, f_, f_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_,
                            p_, i_, p_, p_, i_, p_, p_, p_, i_, p_, p_, p_, p_]),
     "orc_perovskite_sites": (i_, [i_, i_, d_, i_, i_, i_, i_, d_, i_, i_, p_, p_]),
@@ -225,6 +227,36 @@ def multislice(sp, wave_in, var, weight=1.0, prec=32, islc0=0, want_wave=False, 
                             _p(w), _p(v), islc0, weight, _p(oi), _p(ow))
     assert rc == 0
     return oi[:, :nd], ow
+
+
+def multislice_channels(sp, wave_in, var, weight=1.0, prec=32, islc0=0, flags=1 | 4 | 2 | 8 | 16,
+                        chans=None):
+    """One CPUMultislice pass with the channels selected by `flags`
+    (1 integrated, 2 image, 4 diffraction, 8 real-space wave, 16 Fourier-space
+    wave; ReadoutDifDet_h / ReadoutImgDet_h, JMultiSlice.cpp:3369-3538).
+    Accumulates into `chans` (a dict from a previous call) when given.
+    Returns dict: int [ndetslc, ndet], dif/img [ndetslc, ny, nx],
+    wfr/wff [ndetslc, ny, nx] complex."""
+    P = _Packed(sp)
+    nd = len(sp.detectors)
+    rt = np.float32 if prec == 32 else np.float64
+    ct = np.complex64 if prec == 32 else np.complex128
+    shp = (sp.ndetslc, sp.ny, sp.nx)
+    if chans is None:
+        chans = {"int": np.zeros((sp.ndetslc, max(1, nd)), np.float32),
+                 "dif": np.zeros(shp, rt) if flags & 4 else None,
+                 "img": np.zeros(shp, rt) if flags & 2 else None,
+                 "wfr": np.zeros(shp, ct) if flags & 8 else None,
+                 "wff": np.zeros(shp, ct) if flags & 16 else None}
+    w = np.ascontiguousarray(wave_in, np.complex64)
+    v = np.ascontiguousarray(var if len(var) else [0], np.int32)
+    rc = L().orc_multislice_ch(prec, sp.nx, sp.ny, len(sp.gratings), _p(P.nv), _p(P.slcoff), _p(P.pgr),
+                               len(sp.propagators), _p(P.proidx), _p(P.pro), len(sp.objslc), _p(P.objslc),
+                               _p(P.det_objslc), sp.ndetslc, nd, _p(P.det), _p(P.msk), _p(P.msklen), flags,
+                               _p(w), _p(v), islc0, weight, _p(chans["int"]), _p(chans["dif"]),
+                               _p(chans["img"]), _p(chans["wfr"]), _p(chans["wff"]))
+    assert rc == 0
+    return chans
 
 
 def stem_scan(sp, dx, dy, weights, var_all, prec=32, nthreads=1, dz=None, want_wave=False):

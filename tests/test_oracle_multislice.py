@@ -123,3 +123,57 @@ def test_offset_wave_shift_theorem():
     r0 = np.fft.ifft2(sp.probe.astype(np.complex128))
     r1 = np.fft.ifft2(w.astype(np.complex128))
     assert np.abs(r1 - np.roll(np.roll(r0, dy_pix, 0), dx_pix, 1)).max() < 1e-6 * np.abs(r0).max()
+
+
+def numpy_channels(sp, wave_in, var, weight):
+    """float64 numpy evaluation of the intensity and wave channels
+    (ReadoutDifDet_h / ReadoutImgDet_h, JMultiSlice.cpp:3369-3538): Fourier-space
+    channels at each detection plane before slice j, real-space channels
+    after the inverse transform, the exit plane last."""
+    n = sp.nx * sp.ny
+    shp = (sp.ndetslc, sp.ny, sp.nx)
+    ch = {"dif": np.zeros(shp), "img": np.zeros(shp), "wfr": np.zeros(shp, complex), "wff": np.zeros(shp, complex)}
+    w = wave_in.astype(np.complex128)
+
+    def fourier(plane, w):
+        i = sp.det_objslc[plane]
+        if i >= 0:
+            ch["dif"][i] += weight * np.abs(w) ** 2
+            ch["wff"][i] += weight * w
+
+    def real(plane, r):
+        i = sp.det_objslc[plane]
+        if i >= 0:
+            ch["img"][i] += weight * np.abs(r) ** 2
+            ch["wfr"][i] += weight * r
+
+    S = len(sp.objslc)
+    for j in range(S):
+        fourier(j, w)
+        r = np.fft.ifft2(w) * n
+        real(j, r)
+        js = sp.objslc[j] % len(sp.gratings)
+        w = np.fft.fft2(r * sp.gratings[js][var[j]]) * sp.propagators[sp.proidx[js]]
+    fourier(S, w)
+    real(S, np.fft.ifft2(w) * n)
+    return ch
+
+
+@pytest.mark.parametrize("nx,ny,ndetper", [(32, 32, 0), (40, 24, 2)])
+def test_oracle_channels_vs_numpy(nx, ny, ndetper):
+    sp = random_setup(nx, ny, nslices=5, nvar=(2, 3), ndetper=ndetper, seed=nx + 7 * ny, det_ranges=((0, 20),))
+    var = configs.variant_sequences(sp, 1, seed=3)[0]
+    win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.05, -0.03)
+    ref = numpy_channels(sp, win, var, float(np.float32(0.7)))  # the oracle takes weight as float
+    got64 = O.multislice_channels(sp, win, var, 0.7, prec=64)
+    got32 = O.multislice_channels(sp, win, var, 0.7, prec=32)
+    n = nx * ny
+    for k in ("dif", "img", "wfr", "wff"):
+        scale = np.abs(ref[k]).max()
+        assert np.abs(got64[k] - ref[k]).max() / scale < 1e-12, k
+        assert np.abs(got32[k] - ref[k]).max() / scale < 2e-5, k
+    # Parseval ties the planes together: sum |psi(r)|^2 = n sum |psi_k|^2
+    assert np.allclose(ref["img"].sum(axis=(1, 2)), n * ref["dif"].sum(axis=(1, 2)), rtol=1e-10)
+    # accumulation: a second call adds the same amounts
+    again = O.multislice_channels(sp, win, var, 0.7, prec=64, chans=got64)
+    assert np.allclose(again["img"], 2 * ref["img"], rtol=1e-10, atol=1e-12 * np.abs(ref["img"]).max())
