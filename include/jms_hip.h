@@ -63,6 +63,15 @@ int jms_hip_upload_detector(jms_hip_engine* e, int idet, const float* det, int m
                             const int* msk);
 /* incident (probe) wave in Fourier space -- SetIncidentWave :1805 */
 int jms_hip_set_probe(jms_hip_engine* e, const float* wav0);
+/* probe in Fourier space computed on the device into the incident-wave slot:
+ * aperture (ApertureFunctionA, JProbeGen.cpp:471-505) times exp(-i chi) of
+ * the axial aberrations (AberrationFunction :508-571), unit intensity
+ * (CalculateProbeWaveFourier :707-777).  prm: alpha [mrad], edge smoothness,
+ * aperture centre x, y [mrad], asymmetry, asymmetry direction [deg], beam
+ * tilt x, y [mrad]; abrr: nabrr (x, y) coefficient pairs [nm] in the
+ * interface order (image shift, defocus, 2-fold astigmatism, coma, ...);
+ * out: optional host copy (2*nx*ny floats) */
+int jms_hip_probe(jms_hip_engine* e, const float* prm, int nabrr, const float* abrr, float* out);
 /* constant phase plate [rad] (nx*ny floats), NULL = zero -- :1858-1932 */
 int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl);
 

@@ -94,6 +94,7 @@ _SIGS = [
     ("jms_hip_upload_detector", c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
     ("jms_hip_set_probe", c_int, [c_void_p, c_void_p]),
     ("jms_hip_set_phaseplate", c_int, [c_void_p, c_void_p]),
+    ("jms_hip_probe", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("jms_hip_stage_scan", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                    c_void_p]),
     ("jms_hip_run_staged", c_int, [c_void_p, c_int, c_int, c_int]),

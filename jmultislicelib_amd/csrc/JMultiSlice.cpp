@@ -184,18 +184,81 @@ int CJMultiSlice::SetDetectionPlanes(std::vector<int> slc_det, int nobjslc, int*
 // ---------------------------------------------------------------------------
 // calculators
 // ---------------------------------------------------------------------------
+// Axial aberrations of CJProbeGen: the interface list enumerates (order m,
+// rotational symmetry n) for m = 1..8, n = 0..m, m + n even
+// (JProbeGen.cpp:187-216), and the phase is
+//   chi(q) = 2 pi / wl * sum_k Re[(ax - i ay)(wx + i wy)^n] |w|^(m-n) / m,
+// w = wl q, summed term by term in float as AberrationFunction does
+// (JProbeGen.cpp:508-571).
+namespace {
+constexpr int kAbOrderMax = 8;  // _JPG_ABERRATION_ORDER_MAX (JProbeGen.h:53)
+constexpr int kAbNum = (4 * kAbOrderMax - kAbOrderMax % 2 + kAbOrderMax * kAbOrderMax) / 4;
+
+struct AberrationTable {
+  int mn[2 * kAbNum];
+  int binom[(2 * kAbOrderMax + 1) * (2 * kAbOrderMax + 1)];
+  AberrationTable() {
+    int l = 0;
+    for (int m = 1; m <= kAbOrderMax; m++)
+      for (int n = 0; n <= m; n++)
+        if ((m + n) % 2 == 0) { mn[2 * l] = m; mn[2 * l + 1] = n; l++; }
+    const int nol = 2 * kAbOrderMax + 1;
+    for (int k = 0; k < nol; k++)
+      for (int n = 0; n < nol; n++) {  // C(n, k) by the multiplicative formula
+        int res = 0;
+        if (n >= k) {
+          res = 1;
+          const int lk = std::min(k, n - k);
+          for (int i = 0; i < lk; ++i) { res *= (n - i); res /= (i + 1); }
+        }
+        binom[n + k * nol] = res;
+      }
+  }
+};
+
+float aberration_phase(float qx, float qy, float wl, int nabrr, const float* coeff) {
+  static const AberrationTable T;
+  if (nabrr == 0 || !coeff || wl <= 0.f) return 0.f;
+  constexpr int nol = kAbOrderMax + 1, nol2 = 2 * kAbOrderMax + 1;
+  const int nab = std::min(nabrr, kAbNum);
+  const float re_il[4] = {1.f, 0.f, -1.f, 0.f};  // Re(i^l); Im(i^l) = re_il[(l + 3) % 4]
+  const float wx = qx * wl, wy = qy * wl;
+  const float w = std::sqrt(wx * wx + wy * wy);
+  const float prefac = (float)(2.0 * kPI) / wl;
+  float pwr_x[nol], pwr_y[nol], pwr_w[nol];
+  pwr_x[0] = pwr_y[0] = pwr_w[0] = 1.f;
+  for (int i = 1; i < nol; i++) {
+    pwr_x[i] = pwr_x[i - 1] * wx;
+    pwr_y[i] = pwr_y[i - 1] * wy;
+    pwr_w[i] = pwr_w[i - 1] * w;
+  }
+  float sum = 0.f;
+  for (int k = 0; k < nab; k++) {
+    const float cx = coeff[2 * k], cy = coeff[2 * k + 1];
+    if ((cx * cx + cy * cy) > (float)(1.E-15)) {  // _JPG_ABERRATION_STRTHRESH (JProbeGen.h:54)
+      const int m = T.mn[2 * k], n = T.mn[2 * k + 1];
+      float term = 0.f;
+      for (int l = 0; l <= n; l++) {  // binomial expansion of (wx + i wy)^n
+        float c = 0.f;
+        c = c + re_il[l % 4] * cx;
+        c = c + re_il[(l + 3) % 4] * cy;
+        term = term + c * (float)T.binom[n + l * nol2] * pwr_x[n - l] * pwr_y[l];
+      }
+      term = term * pwr_w[m - n] / ((float)m);
+      sum = sum + term;
+    }
+  }
+  return sum * prefac;
+}
+}  // namespace
+
 // CJProbeGen::CalculateProbeWaveFourier (JProbeGen.cpp:707-777) with the
-// aperture of ApertureFunctionA (JProbeGen.cpp:471-505); aberration-free only.
+// aperture of ApertureFunctionA (JProbeGen.cpp:471-505) and the axial
+// aberration phase of m_abrr_num coefficient pairs: psi = A exp(-i chi).
 int CJMultiSlice::CalculateProbeWaveFourier(CJProbeParams* prm, fcmplx* wav) {
   if (!prm || !wav) return 1;
   if (m_wl <= 0.f || m_a0[0] * m_a0[1] <= 0.f || m_nscx * m_nscy <= 0 || prm->m_alpha <= 0.f) return 1;
-  if (prm->m_abrr_coeff && prm->m_abrr_num > 0) {
-    for (int k = 0; k < 2 * prm->m_abrr_num; ++k)
-      if (prm->m_abrr_coeff[k] != 0.f) {
-        std::cerr << "Error: (CalculateProbeWaveFourier): probe aberrations are not implemented." << std::endl;
-        return 2;
-      }
-  }
+  const int nab = (prm->m_abrr_coeff && prm->m_abrr_num > 0) ? prm->m_abrr_num : 0;
   const int nx = m_nscx, ny = m_nscy;
   const float wl = m_wl, ax = m_a0[0], ay = m_a0[1];
   const float qlim = prm->m_alpha * 0.001f / wl;
@@ -226,7 +289,8 @@ int CJMultiSlice::CalculateProbeWaveFourier(CJProbeParams* prm, fcmplx* wav) {
         else if (dqm < qlim) amp = 1.f;
       }
       if (amp > 1.0e-6f) {  // _JPG_PROBE_APERTURE_THRESH (JProbeGen.h:52)
-        wav[(size_t)i + (size_t)j * nx] = fcmplx(amp * std::cos(0.f), -amp * std::sin(0.f));
+        const float chi = nab > 0 ? aberration_phase(qx[i], qy[j], wl, nab, prm->m_abrr_coeff) : 0.f;
+        wav[(size_t)i + (size_t)j * nx] = fcmplx(amp * std::cos(chi), -amp * std::sin(chi));
         wpow += amp * amp;
       }
     }

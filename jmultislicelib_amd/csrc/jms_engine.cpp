@@ -552,6 +552,51 @@ int jms_hip_set_probe(jms_hip_engine* e, const float* wav0) {
   return 0;
 }
 
+int jms_hip_probe(jms_hip_engine* e, const float* prm, int nabrr, const float* abrr, float* out) {
+  if (!e) return 1;
+  if (!prm || e->n <= 0 || e->wl <= 0.f) return e->fail(2, "invalid probe parameters or grid");
+  if (nabrr < 0 || (nabrr > 0 && !abrr)) return e->fail(3, "invalid aberration list");
+  const float wl = e->wl;
+  const float alpha = prm[0], alpha_rs = prm[1], ax0 = prm[2], ay0 = prm[3], asym = prm[4];
+  const float adir = prm[5] * 0.0174533f, btx = prm[6], bty = prm[7];
+  if (alpha <= 0.f) return e->fail(4, "aperture must be open");
+  jms::ProbeArgsHost a;
+  a.nx = e->nx; a.ny = e->ny; a.ax = e->a0x; a.ay = e->a0y; a.wl = wl;
+  a.qlim = alpha * 0.001f / wl;  // mrad -> 1/nm, as CalculateProbeWaveFourier
+  a.qrs = alpha_rs;
+  a.a1x = asym * std::cos(2.f * adir);
+  a.a1y = asym * std::sin(2 * adir);
+  const float adet = 1.f - asym * asym;
+  a.open = adet != 0.f;
+  a.radet = a.open ? 1.f / adet : 1.f;
+  a.qlcx = ax0 * 0.001f / wl; a.qlcy = ay0 * 0.001f / wl;
+  a.btx = btx * 0.001f / wl; a.bty = bty * 0.001f / wl;
+  // aberration list (order m, symmetry n), m = 1..8, m + n even (JProbeGen.cpp:187-216)
+  int l = 0;
+  for (int m = 1; m <= 8; m++)
+    for (int n = 0; n <= m; n++)
+      if ((m + n) % 2 == 0) { a.mn[2 * l] = m; a.mn[2 * l + 1] = n; l++; }
+  for (int n = 0; n <= 8; n++)
+    for (int k = 0; k <= n; k++) {
+      int c = 1;
+      for (int i = 0; i < std::min(k, n - k); ++i) { c *= (n - i); c /= (i + 1); }
+      a.binom[n + 9 * k] = c;
+    }
+  a.nab = std::min(nabrr, 24);
+  for (int k = 0; k < 2 * a.nab; ++k) a.coeff[k] = abrr[k];
+  (void)hipSetDevice(e->device);
+  const size_t nb = sizeof(float) * 2 * (size_t)e->n;
+  DevBuf partial;
+  hipError_t err = e->wav0.ensure(nb);
+  if (err == hipSuccess) err = partial.ensure(sizeof(double) * (size_t)((e->n + 255) / 256));
+  if (err == hipSuccess) err = jms::launch_probe(e->wav0.as<float2>(), partial.as<double>(), a, e->st);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+  if (err == hipSuccess && out) err = hipMemcpy(out, e->wav0.p, nb, hipMemcpyDeviceToHost);
+  partial.release();
+  if (err != hipSuccess) return e->fail(102, "probe on the device", err);
+  return 0;
+}
+
 int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl) {
   if (!e) return 1;
   (void)hipSetDevice(e->device);

@@ -48,6 +48,21 @@ struct RowChanHost {
   long long chan_off = 0;        // offset of this detection plane (items)
 };
 
+// parameters of the device probe generator: spatial frequencies [1/nm]
+// already derived from the probe's angles as the host generator does
+struct ProbeArgsHost {
+  int nx = 0, ny = 0;
+  float ax = 0.f, ay = 0.f, wl = 0.f;           // cell [nm], wavelength [nm]
+  float qlim = 0.f, qrs = 0.f;                  // aperture radius, edge smoothness
+  float a1x = 0.f, a1y = 0.f, radet = 1.f;      // aperture distortion
+  int open = 1;                                 // 0: aperture closed by distortion
+  float qlcx = 0.f, qlcy = 0.f, btx = 0.f, bty = 0.f;  // aperture centre, beam tilt
+  int nab = 0;                                  // aberration pairs used
+  float coeff[48] = {};                         // (x, y) per aberration [nm]
+  int mn[48] = {};                              // (order, symmetry) per aberration
+  int binom[81] = {};                           // C(n, l) at n + 9 l, n, l <= 8
+};
+
 constexpr int kMaxDetPerSweep = 8;
 
 bool size_supported(int n);
@@ -74,5 +89,7 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
                               float kscy, float dx, float dy, float fz, const float* cppl,
                               int ish0, int ish1, hipStream_t st);
 hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st);
+// probe into out (n complex), partial: ceil(n/256) doubles of scratch
+hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st);
 
 }  // namespace jms

@@ -76,6 +76,97 @@ __global__ void k_offset_wave(float2* __restrict__ out, const float2* __restrict
   out[idx] = cmul(in[(long long)i0 + (long long)j0 * nx], make_float2(cs, sn));
 }
 
+// ---------------------------------------------------------------------------
+// probe in Fourier space on the device: the aperture of ApertureFunctionA
+// (JProbeGen.cpp:471-505) times exp(-i chi) with the axial aberration phase
+// of AberrationFunction (JProbeGen.cpp:508-571), then normalised to unit
+// intensity (CalculateProbeWaveFourier, JProbeGen.cpp:707-777).
+// ---------------------------------------------------------------------------
+__device__ float probe_chi(float qx, float qy, const ProbeArgsHost& a) {
+  const float wx = qx * a.wl, wy = qy * a.wl;
+  const float w = sqrtf(wx * wx + wy * wy);
+  float px[9], py[9], pw[9];
+  px[0] = py[0] = pw[0] = 1.f;
+#pragma unroll
+  for (int i = 1; i < 9; ++i) { px[i] = px[i - 1] * wx; py[i] = py[i - 1] * wy; pw[i] = pw[i - 1] * w; }
+  const float re_il[4] = {1.f, 0.f, -1.f, 0.f};
+  float sum = 0.f;
+  for (int k = 0; k < a.nab; ++k) {
+    const float cx = a.coeff[2 * k], cy = a.coeff[2 * k + 1];
+    if (cx * cx + cy * cy > 1.0e-15f) {
+      const int m = a.mn[2 * k], n = a.mn[2 * k + 1];
+      float term = 0.f;
+      for (int l = 0; l <= n; ++l) {
+        float c = 0.f;
+        c = c + re_il[l & 3] * cx;
+        c = c + re_il[(l + 3) & 3] * cy;
+        term = term + c * (float)a.binom[n + 9 * l] * px[n - l] * py[l];
+      }
+      sum = sum + term * pw[m - n] / (float)m;
+    }
+  }
+  return sum * (6.28318530717958648f / a.wl);
+}
+
+__global__ void k_probe(float2* __restrict__ out, double* __restrict__ partial, ProbeArgsHost a) {
+  __shared__ double red[256];
+  const long long n = (long long)a.nx * a.ny;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double pw = 0.0;
+  if (idx < n) {
+    const int i = (int)(idx % a.nx), j = (int)(idx / a.nx);
+    const float qx = (float)fidx(i, a.nx) / a.ax, qy = (float)fidx(j, a.ny) / a.ay;
+    float amp = 0.f;
+    if (a.qlim > 0.f && a.open) {
+      const float dqx = (qx - a.btx) - a.qlcx, dqy = (qy - a.bty) - a.qlcy;
+      const float ux = ((1.f - a.a1x) * dqx - a.a1y * dqy) * a.radet;
+      const float uy = (-a.a1y * dqx + (1.f + a.a1x) * dqy) * a.radet;
+      const float dqm = sqrtf(ux * ux + uy * uy);
+      const float dqs = fabsf(a.qrs * a.qlim);
+      if (dqm <= 0.f) amp = 1.f;
+      else if (dqs > 0.f) amp = (1.f - tanhf(3.14159265358979324f * (dqm - a.qlim) / dqs)) * 0.5f;
+      else if (dqm < a.qlim) amp = 1.f;
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (amp > 1.0e-6f) {
+      const float chi = a.nab > 0 ? probe_chi(qx, qy, a) : 0.f;
+      float sn, cs;
+      sincosf(chi, &sn, &cs);
+      v = make_float2(amp * cs, -amp * sn);
+      pw = (double)(amp * amp);
+    }
+    out[idx] = v;
+  }
+  red[threadIdx.x] = pw;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_probe_scale(float2* __restrict__ out, const double* __restrict__ partial, int nblk,
+                              long long n) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const float fsca = 1.f / sqrtf((float)red[0]);
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) {
+    float2 v = out[idx];
+    v.x *= fsca;
+    v.y *= fsca;
+    out[idx] = v;
+  }
+}
+
 // dst += w * src (complex), n items
 __global__ void k_axpy_c(float2* __restrict__ dst, const float2* __restrict__ src, long long n, float w) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,6 +279,16 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
   const long long n = (long long)nx * ny;
   hipLaunchKernelGGL(k_offset_wave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, in,
                      nx, ny, kscx, kscy, dx, dy, fz, cppl, ish0, ish1);
+  return hipGetLastError();
+}
+
+hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st) {
+  const long long n = (long long)a.nx * a.ny;
+  const unsigned nblk = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_probe, dim3(nblk), dim3(256), 0, st, out, partial, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_probe_scale, dim3(nblk), dim3(256), 0, st, out, (const double*)partial, (int)nblk, n);
   return hipGetLastError();
 }
 

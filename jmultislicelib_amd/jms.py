@@ -107,9 +107,17 @@ class JMultiSlice:
 
     def CalculateProbeWaveFourier(self, alpha_mrad: float, alpha_rs: float = 0.03, alpha_x0=0.0,
                                   alpha_y0=0.0, alpha_asym=0.0, alpha_adir=0.0, btx=0.0,
-                                  bty=0.0) -> np.ndarray:
+                                  bty=0.0, abrr=None) -> np.ndarray:
+        """Probe in Fourier space; abrr: [k][x, y] axial aberration coefficients
+        [nm] in the library's interface order (image shift, defocus, 2-fold
+        astigmatism, coma, 3-fold astigmatism, spherical, ...)."""
         nx, ny = self.GetGridSize()
         prm = CJProbeParams()
+        ab = None
+        if abrr is not None:
+            ab = np.ascontiguousarray(abrr, np.float32).reshape(-1)
+            prm.m_abrr_coeff = ab.ctypes.data_as(C.POINTER(C.c_float))
+            prm.m_abrr_num = len(ab) // 2
         prm.m_alpha = alpha_mrad
         prm.m_alpha_rs = alpha_rs
         prm.m_alpha_x0 = alpha_x0
@@ -118,7 +126,8 @@ class JMultiSlice:
         prm.m_alpha_adir = alpha_adir
         prm.m_btx = btx
         prm.m_bty = bty
-        prm.m_abrr_num = 0
+        if ab is None:
+            prm.m_abrr_num = 0
         wav = np.zeros((ny, nx), np.complex64)
         _chk("CalculateProbeWaveFourier", self.L.CalculateProbeWaveFourier(C.byref(prm), ptr(wav)))
         return wav
@@ -293,6 +302,16 @@ class Engine:
         self._c("jms_hip_set_probe", ptr(w0))
         self.sp = sp
         return self
+
+    def probe(self, alpha_mrad, alpha_rs=0.03, alpha_x0=0.0, alpha_y0=0.0, alpha_asym=0.0, alpha_adir=0.0,
+              btx=0.0, bty=0.0, abrr=None) -> np.ndarray:
+        """Generate the incident probe on the device (it becomes the scan's
+        incoming wave) and return a host copy."""
+        prm = _f32([alpha_mrad, alpha_rs, alpha_x0, alpha_y0, alpha_asym, alpha_adir, btx, bty])
+        ab = None if abrr is None else _f32(np.asarray(abrr, np.float32).reshape(-1))
+        out = np.zeros((self.sp.ny, self.sp.nx), np.complex64)
+        self._c("jms_hip_probe", ptr(prm), 0 if ab is None else len(ab) // 2, ptr(ab), ptr(out))
+        return out
 
     def stage_scan(self, dx, dy, dz, weights, var):
         dx, dy = _f32(dx), _f32(dy)

@@ -584,14 +584,110 @@ ORC_API int orc_ring_detector(int nx, int ny, float a0x, float a0y, float wlf, i
   return 0;
 }
 
+// Axial aberration table of CJProbeGen (JProbeGen.cpp:187-245): aberration k
+// of the interface list has order m and rotational symmetry n, enumerated
+// for m = 1..8, n = 0..m with m + n even: (1,1) image shift, (2,0) defocus,
+// (2,2) 2-fold astigmatism, (3,1) coma, (3,3), (4,0) spherical, ...
+static const int kAbOrderMax = 8;  // _JPG_ABERRATION_ORDER_MAX (JProbeGen.h:53)
+struct AbTable {
+  int num = 0;
+  std::vector<int> widx;   // (m, n) pairs
+  std::vector<int> binom;  // binom[n + k*nol2] = C(n, k), nol2 = 2*8+1
+  AbTable() {
+    num = (4 * kAbOrderMax - kAbOrderMax % 2 + kAbOrderMax * kAbOrderMax) / 4;
+    widx.assign(2 * num, 0);
+    int l = 0;
+    for (int m = 1; m <= kAbOrderMax; m++)
+      for (int n = 0; n <= m; n++)
+        if ((m + n) % 2 == 0) { widx[2 * l] = m; widx[2 * l + 1] = n; l++; }
+    const int nol = 1 + 2 * kAbOrderMax;
+    binom.assign(nol * nol, 0);
+    for (int k = 0; k < nol; k++)
+      for (int n = 0; n < nol; n++) {
+        int res = 0;
+        if (n >= k) {
+          res = 1;
+          const int lk = (k > n - k) ? n - k : k;
+          for (int i = 0; i < lk; ++i) { res *= (n - i); res /= (i + 1); }
+        }
+        binom[n + k * nol] = res;
+      }
+  }
+};
+static const AbTable& ab_table() { static const AbTable t; return t; }
+
+// CJProbeGen::AberrationFunction (JProbeGen.cpp:508-571): the phase chi [rad]
+// at spatial frequency (qx, qy) [1/nm] for nabrr coefficient pairs (x, y) [nm]
+// of the interface list: chi = 2pi/wl * sum_k Re[(ax - i ay)(wx + i wy)^n]
+// |w|^(m-n) / m with w = wl * q, evaluated term by term in float.
+static float aberration_chi(float qx, float qy, float wl, int nabrr, const float* coeff) {
+  if (nabrr == 0 || !coeff || wl <= 0.f) return 0.f;
+  const AbTable& T = ab_table();
+  const int nol = kAbOrderMax + 1, nol2 = kAbOrderMax * 2 + 1;
+  const int nab = std::min(nabrr, T.num);
+  const float sgnprm[4] = {1.f, 0.f, -1.f, 0.f};
+  const float wx = qx * wl, wy = qy * wl;
+  const float w = std::sqrt(wx * wx + wy * wy);
+  const float prefac = (float)(2.0 * kPI) / wl;
+  std::vector<float> wfield(2 * nol), wabs(nol);
+  float pwx = 1.f, pwy = 1.f, pw = 1.f;
+  wfield[0] = pwx; wfield[1] = pwy; wabs[0] = pw;
+  for (int i = 1; i < nol; i++) {
+    pwx *= wx; pwy *= wy; pw *= w;
+    wfield[2 * i] = pwx; wfield[2 * i + 1] = pwy; wabs[i] = pw;
+  }
+  float rtmp = 0.f;
+  for (int k = 0; k < nab; k++) {
+    const float wax = coeff[2 * k], way = coeff[2 * k + 1];
+    if ((wax * wax + way * way) > (float)(1.E-15)) {  // _JPG_ABERRATION_STRTHRESH (JProbeGen.h:54)
+      const int m = T.widx[2 * k], n = T.widx[2 * k + 1];
+      float ttmp = 0.f;
+      for (int l = 0; l <= n; l++) {
+        float ttmp1 = 0.f;
+        ttmp1 = ttmp1 + sgnprm[l % 4] * wax;
+        ttmp1 = ttmp1 + sgnprm[(l + 3) % 4] * way;
+        const int j = n - l;
+        ttmp = ttmp + ttmp1 * (float)T.binom[n + l * nol2] * wfield[2 * j] * wfield[2 * l + 1];
+      }
+      ttmp = ttmp * wabs[m - n] / ((float)m);
+      rtmp = rtmp + ttmp;
+    }
+  }
+  return rtmp * prefac;
+}
+
+ORC_API float orc_aberration_chi(float qx, float qy, float wl, int nabrr, const float* coeff) {
+  return aberration_chi(qx, qy, wl, nabrr, coeff);
+}
+
+ORC_API int orc_aberration_table(int* num, int* widx) {
+  const AbTable& T = ab_table();
+  if (num) *num = T.num;
+  if (widx) std::copy(T.widx.begin(), T.widx.end(), widx);
+  return 0;
+}
+
 // CJProbeGen::CalculateProbeWaveFourier (JProbeGen.cpp:707-777) with
-// ApertureFunctionA (JProbeGen.cpp:471-505), for zero aberrations (the phase
-// chi is 0, so each in-aperture pixel is (wap, -0)).  Angles in mrad, adir in
-// degrees.  wav: 2*nx*ny floats.
+// ApertureFunctionA (JProbeGen.cpp:471-505) and the aberration phase chi of
+// nabrr coefficient pairs (none: chi = 0).  Angles in mrad, adir in degrees.
+// wav: 2*nx*ny floats.
+ORC_API int orc_probe_wave_ab(int nx, int ny, float ax, float ay, float wl, float alpha,
+                              float alpha_rs, float alpha_asym, float alpha_adir,
+                              float alpha_x0, float alpha_y0, float btx_mrad, float bty_mrad,
+                              int nabrr, const float* abrr, float* wav);
+
 ORC_API int orc_probe_wave(int nx, int ny, float ax, float ay, float wl, float alpha,
                            float alpha_rs, float alpha_asym, float alpha_adir,
                            float alpha_x0, float alpha_y0, float btx_mrad, float bty_mrad,
                            float* wav) {
+  return orc_probe_wave_ab(nx, ny, ax, ay, wl, alpha, alpha_rs, alpha_asym, alpha_adir, alpha_x0, alpha_y0,
+                           btx_mrad, bty_mrad, 0, nullptr, wav);
+}
+
+ORC_API int orc_probe_wave_ab(int nx, int ny, float ax, float ay, float wl, float alpha,
+                              float alpha_rs, float alpha_asym, float alpha_adir,
+                              float alpha_x0, float alpha_y0, float btx_mrad, float bty_mrad,
+                              int nabrr, const float* abrr, float* wav) {
   if (!wav || wl <= 0.f || ax * ay <= 0.f || nx * ny <= 0 || alpha <= 0.f) return 1;
   const int nx2 = (nx - (nx % 2)) >> 1, ny2 = (ny - (ny % 2)) >> 1;
   const float qlim = alpha * 0.001f / wl;
@@ -605,9 +701,9 @@ ORC_API int orc_probe_wave(int nx, int ny, float ax, float ay, float wl, float a
   for (int i = 0; i < nx; i++) qnx[i] = (float)(((i + nx2) % nx) - nx2) / ax;
   for (int j = 0; j < ny; j++) qny[j] = (float)(((j + ny2) % ny) - ny2) / ay;
   float wpow = 0.f;
-  const float chi = 0.f;
   for (int j = 0; j < ny; j++) {
     for (int i = 0; i < nx; i++) {
+      float chi = 0.f;
       // ApertureFunctionA(qnx[i]-btx, qny[j]-bty, qlcx, qlcy, qlim, alim, adir, qrs)
       float wap = 0.f;
       const float qx = qnx[i] - btx, qy = qny[j] - bty;
@@ -636,6 +732,7 @@ ORC_API int orc_probe_wave(int nx, int ny, float ax, float ay, float wl, float a
         }
       }
       if (wap > (float)(1.E-6)) {  // _JPG_PROBE_APERTURE_THRESH, JProbeGen.h:52
+        if (nabrr > 0) chi = aberration_chi(qnx[i], qny[j], wl, nabrr, abrr);  // axial, untilted q
         float* o = wav + 2 * ((size_t)i + (size_t)j * nx);
         o[0] = wap * std::cos(chi);
         o[1] = -wap * std::sin(chi);

@@ -25,6 +25,9 @@ _SIGS = {
     "orc_propagator": (i_, [i_, i_, f_, f_, f_, f_, f_, f_, i_, p_]),
     "orc_ring_detector": (i_, [i_, i_, f_, f_, f_, i_, f_, f_, f_, f_, f_, f_, p_, i_, f_, p_, p_, p_, i_]),
     "orc_probe_wave": (i_, [i_, i_, f_, f_, f_, f_, f_, f_, f_, f_, f_, f_, f_, p_]),
+    "orc_probe_wave_ab": (i_, [i_, i_, f_, f_, f_, f_, f_, f_, f_, f_, f_, f_, f_, i_, p_, p_]),
+    "orc_aberration_chi": (f_, [f_, f_, f_, i_, p_]),
+    "orc_aberration_table": (i_, [p_, p_]),
     "orc_detection_planes": (i_, [i_, i_, p_]),
     "orc_rng_create": (p_, [i_]),
     "orc_rng_destroy": (None, [p_]),
@@ -114,11 +117,29 @@ def ring_detector(nx, ny, a0x, a0y, wl, nprotype, beta0, beta1, phi0=0.0, phi1=0
 
 
 def probe_wave(nx, ny, ax, ay, wl, alpha, alpha_rs=0.03, alpha_asym=0.0, alpha_adir=0.0, alpha_x0=0.0,
-               alpha_y0=0.0, btx=0.0, bty=0.0):
+               alpha_y0=0.0, btx=0.0, bty=0.0, abrr=None):
+    """Probe in Fourier space (JProbeGen.cpp:707-777); abrr: [k][x, y] aberration
+    coefficients [nm] in the order of aberration_table()."""
     out = np.zeros((ny, nx), np.complex64)
-    assert L().orc_probe_wave(nx, ny, ax, ay, wl, alpha, alpha_rs, alpha_asym, alpha_adir, alpha_x0, alpha_y0,
-                              btx, bty, _p(out)) == 0
+    ab = None if abrr is None else np.ascontiguousarray(abrr, np.float32).reshape(-1)
+    nab = 0 if ab is None else len(ab) // 2
+    assert L().orc_probe_wave_ab(nx, ny, ax, ay, wl, alpha, alpha_rs, alpha_asym, alpha_adir, alpha_x0,
+                                 alpha_y0, btx, bty, nab, _p(ab), _p(out)) == 0
     return out
+
+
+def aberration_chi(qx, qy, wl, abrr):
+    ab = np.ascontiguousarray(abrr, np.float32).reshape(-1)
+    return L().orc_aberration_chi(qx, qy, wl, len(ab) // 2, _p(ab))
+
+
+def aberration_table():
+    """[(order m, rotational symmetry n)] of the aberration interface list."""
+    num = C.c_int(0)
+    L().orc_aberration_table(C.byref(num), None)
+    w = np.zeros(2 * num.value, np.int32)
+    L().orc_aberration_table(None, _p(w))
+    return [tuple(x) for x in w.reshape(-1, 2)]
 
 
 def detection_planes(ndetper, nobjslc):

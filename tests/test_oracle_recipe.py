@@ -196,3 +196,66 @@ def test_variant_sequence_properties():
     # different seeds give different sequences, same seed the same
     assert not np.array_equal(seqs, O.variant_sequences(18, nv, obj, 5))
     assert np.array_equal(seqs, O.variant_sequences(17, nv, obj, 5))
+
+
+def test_aberration_table_order():
+    """Interface list (JProbeGen.cpp:187-216): orders 1..8, symmetries of equal
+    parity, in that nesting."""
+    t = O.aberration_table()
+    assert len(t) == 24
+    assert t[:10] == [(1, 1), (2, 0), (2, 2), (3, 1), (3, 3), (4, 0), (4, 2), (4, 4), (5, 1), (5, 3)]
+    assert t[-1] == (8, 8) and all((m + n) % 2 == 0 and 0 <= n <= m for m, n in t)
+
+
+def _chi_ref(qx, qy, wl, abrr):
+    """Closed form in float64: chi = 2pi/wl sum Re[(ax - i ay)(wx + i wy)^n] |w|^(m-n) / m."""
+    wx, wy = wl * qx, wl * qy
+    w = np.hypot(wx, wy)
+    chi = 0.0
+    for (m, n), (ax, ay) in zip(O.aberration_table(), np.asarray(abrr, float).reshape(-1, 2)):
+        chi += ((ax - 1j * ay) * (wx + 1j * wy) ** n).real * w ** (m - n) / m
+    return 2 * np.pi / wl * chi
+
+
+def test_aberration_closed_forms():
+    wl = 0.0025079
+    rng = np.random.default_rng(9)
+    for _ in range(50):
+        qx, qy = rng.uniform(-8, 8, 2)  # 1/nm, up to ~20 mrad
+        # defocus C1: pi wl C1 q^2
+        c1 = rng.uniform(-50, 50)
+        ab = np.zeros((24, 2)); ab[1] = (c1, 0)
+        ref = np.pi * wl * c1 * (qx * qx + qy * qy)
+        assert abs(O.aberration_chi(qx, qy, wl, ab) - ref) <= 1e-5 * max(1.0, abs(ref))
+        # image shift (a, b): 2 pi (a qx + b qy)
+        a, b = rng.uniform(-0.3, 0.3, 2)
+        ab = np.zeros((24, 2)); ab[0] = (a, b)
+        ref = 2 * np.pi * (a * qx + b * qy)
+        assert abs(O.aberration_chi(qx, qy, wl, ab) - ref) <= 1e-5 * max(1.0, abs(ref))
+        # spherical C3: pi/2 C3 wl^3 q^4
+        c3 = rng.uniform(-2e6, 2e6)
+        ab = np.zeros((24, 2)); ab[5] = (c3, 0)
+        ref = np.pi / 2 * c3 * wl ** 3 * (qx * qx + qy * qy) ** 2
+        assert abs(O.aberration_chi(qx, qy, wl, ab) - ref) <= 1e-5 * max(1.0, abs(ref))
+        # random set of all 24 (sizes scaled by order)
+        ab = np.array([rng.uniform(-1, 1, 2) * 10.0 ** (m - 1) for m, n in O.aberration_table()])
+        ref = _chi_ref(qx, qy, wl, ab)
+        assert abs(O.aberration_chi(qx, qy, wl, ab) - ref) <= 2e-5 * max(1.0, abs(ref))
+
+
+def test_aberrated_probe_phase_and_shift():
+    nx = ny = 64
+    a0, wl = 3.124, 0.0025079
+    ab = np.zeros((24, 2)); ab[1] = (12.0, 0); ab[2] = (3.0, -2.0); ab[5] = (1.0e5, 0)
+    p0 = O.probe_wave(nx, ny, a0, a0, wl, 25.0, 0.03)
+    p = O.probe_wave(nx, ny, a0, a0, wl, 25.0, 0.03, abrr=ab)
+    assert np.allclose(np.abs(p), np.abs(p0), atol=1e-7)      # aberrations change only the phase
+    k2, fx, fy = _k2(nx, ny, a0, a0)
+    m = np.abs(p0) > 0
+    chi = np.array([[_chi_ref(fx[i], fy[j], wl, ab) for i in range(nx)] for j in range(ny)])
+    assert np.abs(np.angle(p[m] * np.exp(1j * chi[m]))).max() < 1e-3  # psi = A exp(-i chi)
+    # an image shift by whole pixels translates the real-space probe
+    s = np.zeros((24, 2)); s[0] = (3 * a0 / nx, -2 * a0 / ny)
+    ps = O.probe_wave(nx, ny, a0, a0, wl, 25.0, 0.03, abrr=s)
+    r0, r1 = np.fft.ifft2(p0.astype(complex)), np.fft.ifft2(ps.astype(complex))
+    assert np.abs(r1 - np.roll(np.roll(r0, -2, 0), 3, 1)).max() < 1e-5 * np.abs(r0).max()

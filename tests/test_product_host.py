@@ -148,3 +148,16 @@ def test_frozen_phonon_displacement_statistics():
 def test_cpu_code_is_rejected():
     M = _setup()
     assert M.PhaseGratingSetup(1, 64, 48, 1, 1, [1]) != 0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_aberrated_probe_bit_exact(seed):
+    rng = np.random.default_rng(seed)
+    from oracle.oracle import aberration_table
+    ab = np.array([rng.uniform(-1, 1, 2) * 10.0 ** (m - 1) for m, n in aberration_table()], np.float32)
+    ab[rng.random(24) < 0.5] = 0.0  # a sparse set, as measured aberration lists are
+    M = _setup()
+    p = M.CalculateProbeWaveFourier(22.0, 0.03, 0.0, 0.0, 0.05, 15.0, 0.5, -0.5, abrr=ab)
+    r = O.probe_wave(64, 48, 1.7, 1.3, M.GetWaveLength(), 22.0, 0.03, 0.05, 15.0, 0.0, 0.0, 0.5, -0.5, abrr=ab)
+    assert np.array_equal(p.view(np.uint32), r.view(np.uint32))
+    assert abs(np.sum(np.abs(p.astype(np.complex128)) ** 2) - 1.0) < 1e-5
