@@ -48,6 +48,19 @@ int jms_hip_set_grid(jms_hip_engine* e, int nx, int ny, float a0x, float a0y, fl
 int jms_hip_setup_gratings(jms_hip_engine* e, int npgx, int npgy, int nslc, const int* nvarslc);
 /* upload nvar variants of structure slice islc -- SetPhaseGratingData :1440 */
 int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* pgr);
+/* generate nvar variants of structure slice islc on the device, in place of
+ * an upload: phase gratings t = exp(i sig_int V) of the Gaussian
+ * projected-potential model V(r) = S/(2 pi s^2) exp(-|r - ra|^2/(2 s^2)),
+ * S = 2.0 Z^0.7 [V A^2], periodic over the grid cell, images within 8 s
+ * (the host generator of jms_specimen.h, evaluated in the same order).
+ * nat[v] atoms per variant, their positions xy (2 doubles per atom [A]) and
+ * atomic numbers z concatenated over the variants; sigma [A]; sig_int
+ * [rad / (V A)].  No counterpart in the reference library, which takes
+ * gratings from its caller. */
+int jms_hip_gaussian_gratings(jms_hip_engine* e, int islc, int nvar, const int* nat, const double* xy,
+                              const int* z, double sigma, double sig_int);
+/* copy nvar variants of structure slice islc (2*nx*ny floats each) */
+int jms_hip_get_gratings(jms_hip_engine* e, int islc, int nvar, float* out);
 /* npro propagators, proidx[nslc] selects one per structure slice -- :542 */
 int jms_hip_setup_propagators(jms_hip_engine* e, int npro, int nslc, const int* proidx);
 int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro);

@@ -87,6 +87,9 @@ _SIGS = [
     ("jms_hip_set_grid", c_int, [c_void_p, c_int, c_int, c_float, c_float, c_float]),
     ("jms_hip_setup_gratings", c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     ("jms_hip_upload_gratings", c_int, [c_void_p, c_int, c_int, c_void_p]),
+    ("jms_hip_gaussian_gratings", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double,
+                                          c_double]),
+    ("jms_hip_get_gratings", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("jms_hip_setup_propagators", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("jms_hip_upload_propagator", c_int, [c_void_p, c_int, c_void_p]),
     ("jms_hip_setup_objects", c_int, [c_void_p, c_int, c_void_p]),

@@ -188,10 +188,6 @@ struct TimedLaunch {
   }
 };
 
-// Launch sequence of one batch.  `ppar` (float4 per wave) and `var` (nobjslc
-// ints per wave) describe the waves; src/src_stride the INIT source.
-// Readout partial sums go to e->partial, combined into `res` (npos results)
-// when npass > 0.
 // per-wave readout channels of a batch (any may be null; cwgt is the
 // per-wave weight, the channels hold ndetslc planes of n items per wave)
 struct BatchChans {
@@ -207,6 +203,10 @@ struct BatchChans {
   bool real_space() const { return img || img_avg || wfr || wfr_avg; }
 };
 
+// Launch sequence of one batch.  `ppar` (float4 per wave) and `var` (nobjslc
+// ints per wave) describe the waves; src/src_stride the INIT source.
+// Readout partial sums go to e->partial, combined into `res` (npos results)
+// when npass > 0; `ch` are the single-wave channels.
 int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long long src_stride,
                   const void* ppar, const int* var, int npos, int npass, const float* weights,
                   float* res, int accumulate, const BatchChans& ch, bool timed) {
@@ -433,6 +433,88 @@ int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* 
   (void)hipSetDevice(e->device);
   hipError_t err = hipMemcpy(e->pgr.as<float2>() + e->slcoffset[islc], pgr, nb, hipMemcpyHostToDevice);
   if (err != hipSuccess) return e->fail(102, "upload phase gratings", err);
+  return 0;
+}
+
+int jms_hip_gaussian_gratings(jms_hip_engine* e, int islc, int nvar, const int* nat, const double* xy,
+                              const int* z, double sigma, double sig_int) {
+  if (!e) return 1;
+  if (islc < 0 || islc >= e->nscslc || nvar < 1 || !nat || sigma <= 0.0) return e->fail(2, "invalid grating setup");
+  if (e->npgx != e->nx || e->npgy != e->ny) return e->fail(3, "gratings must match the grid");
+  const int nl = std::min(nvar, e->nvarslc[islc]);
+  const int nx = e->nx, ny = e->ny;
+  const double PI = 3.14159265358979323846264338327950288;
+  const double ax = 10.0 * (double)e->a0x, ay = 10.0 * (double)e->a0y;  // nm -> A
+  const double hx = ax / nx, hy = ay / ny;
+  const double rc = 8.0 * sigma, rc2 = rc * rc;
+  const double inv2s2 = 1.0 / (2.0 * sigma * sigma);
+  const int ntx = (nx + 15) / 16, nty = (ny + 15) / 16, ntiles = ntx * nty;
+  // atom records as the host generator computes them, and per-set tile lists
+  std::vector<jms::GaussAtom> atoms;
+  std::vector<int> off((size_t)nl * (ntiles + 1), 0), idx;
+  std::vector<std::vector<int>> tiles_of;
+  size_t a_base = 0;
+  for (int v = 0; v < nl; ++v) {
+    if (nat[v] < 0 || (nat[v] > 0 && (!xy || !z))) return e->fail(4, "invalid atom list");
+    std::vector<std::vector<int>> lists(ntiles);
+    std::vector<char> colhit(ntx), rowhit(nty);
+    for (int at = 0; at < nat[v]; ++at) {
+      const size_t k = a_base + at;
+      jms::GaussAtom g;
+      g.amp = 2.0 * std::pow((double)z[k], 0.7) / (2.0 * PI * sigma * sigma);
+      g.xa = std::fmod(xy[2 * k], ax);
+      if (g.xa < 0) g.xa += ax;
+      g.ya = std::fmod(xy[2 * k + 1], ay);
+      if (g.ya < 0) g.ya += ay;
+      g.i0 = (int)std::floor((g.xa - rc) / hx); g.i1 = (int)std::ceil((g.xa + rc) / hx);
+      g.j0 = (int)std::floor((g.ya - rc) / hy); g.j1 = (int)std::ceil((g.ya + rc) / hy);
+      const int id = (int)atoms.size();
+      atoms.push_back(g);
+      std::fill(colhit.begin(), colhit.end(), 0);
+      std::fill(rowhit.begin(), rowhit.end(), 0);
+      for (int i = g.i0; i <= g.i1 && i < g.i0 + nx; ++i) colhit[(((i % nx) + nx) % nx) / 16] = 1;
+      for (int j = g.j0; j <= g.j1 && j < g.j0 + ny; ++j) rowhit[(((j % ny) + ny) % ny) / 16] = 1;
+      for (int ty = 0; ty < nty; ++ty)
+        if (rowhit[ty])
+          for (int tx = 0; tx < ntx; ++tx)
+            if (colhit[tx]) lists[ty * ntx + tx].push_back(id);
+    }
+    int* o = off.data() + (size_t)v * (ntiles + 1);
+    for (int t = 0; t < ntiles; ++t) {
+      o[t] = (int)idx.size();
+      idx.insert(idx.end(), lists[t].begin(), lists[t].end());
+    }
+    o[ntiles] = (int)idx.size();
+    a_base += nat[v];
+  }
+  if (idx.empty()) idx.push_back(0);
+  if (atoms.empty()) atoms.push_back(jms::GaussAtom{});
+  (void)hipSetDevice(e->device);
+  DevBuf d_atoms, d_off, d_idx;
+  hipError_t err = d_atoms.ensure(sizeof(jms::GaussAtom) * atoms.size());
+  if (err == hipSuccess) err = d_off.ensure(sizeof(int) * off.size());
+  if (err == hipSuccess) err = d_idx.ensure(sizeof(int) * idx.size());
+  if (err == hipSuccess) err = hipMemcpy(d_atoms.p, atoms.data(), sizeof(jms::GaussAtom) * atoms.size(), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_idx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = jms::launch_gauss_gratings(e->pgr.as<float2>() + e->slcoffset[islc], nx, ny, nl, hx, hy, rc2, inv2s2,
+                                     sig_int, (const jms::GaussAtom*)d_atoms.p, d_off.as<int>(), d_idx.as<int>(),
+                                     e->st);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+  d_atoms.release(); d_off.release(); d_idx.release();
+  if (err != hipSuccess) return e->fail(102, "phase gratings on the device", err);
+  return 0;
+}
+
+int jms_hip_get_gratings(jms_hip_engine* e, int islc, int nvar, float* out) {
+  if (!e || !out) return 1;
+  if (islc < 0 || islc >= e->nscslc || nvar < 1) return e->fail(2, "invalid grating read");
+  const int nl = std::min(nvar, e->nvarslc[islc]);
+  (void)hipSetDevice(e->device);
+  hipError_t err = hipMemcpy(out, e->pgr.as<float2>() + e->slcoffset[islc],
+                             sizeof(float) * 2 * (size_t)nl * e->npgx * e->npgy, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->fail(102, "read phase gratings", err);
   return 0;
 }
 

@@ -63,6 +63,14 @@ struct ProbeArgsHost {
   int binom[81] = {};                           // C(n, l) at n + 9 l, n, l <= 8
 };
 
+// one atom of the Gaussian projected-potential model, prepared on the host:
+// wrapped position [A], amplitude S/(2 pi s^2) [V A], and the unwrapped pixel
+// range [i0, i1] x [j0, j1] its 8 s disc covers
+struct GaussAtom {
+  double xa, ya, amp;
+  int i0, i1, j0, j1;
+};
+
 constexpr int kMaxDetPerSweep = 8;
 
 bool size_supported(int n);
@@ -89,6 +97,12 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
                               float kscy, float dx, float dy, float fz, const float* cppl,
                               int ish0, int ish1, hipStream_t st);
 hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, hipStream_t st);
+// phase gratings exp(i sig_int V) of nvar atom sets into out + v * n: tiles of
+// 16 x 16 pixels, atoms of tile t of set v at idx[off[v * (ntiles + 1) + t] ..
+// off[v * (ntiles + 1) + t + 1]) in ascending atom order
+hipError_t launch_gauss_gratings(float2* out, int nx, int ny, int nvar, double hx, double hy,
+                                 double rc2, double inv2s2, double sig_int, const GaussAtom* atoms,
+                                 const int* off, const int* idx, hipStream_t st);
 // probe into out (n complex), partial: ceil(n/256) doubles of scratch
 hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st);
 

@@ -167,6 +167,47 @@ __global__ void k_probe_scale(float2* __restrict__ out, const double* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// phase gratings of the Gaussian projected-potential model on the device
+// (the host generator jms_gaussian_phase_grating, evaluated per pixel): each
+// pixel adds amp * exp(-r^2 / (2 s^2)) of every atom image within the cut-off
+// in the host's order (atoms ascending; per atom, pixel rows then columns of
+// the unwrapped range), in double, then t = exp(i sig_int V).
+// grid: (ceil(nx/16), ceil(ny/16), nvar), block: 16 x 16
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gauss_gratings(float2* __restrict__ out, int nx, int ny, double hx,
+                                                        double hy, double rc2, double inv2s2, double sig_int,
+                                                        const GaussAtom* __restrict__ atoms,
+                                                        const int* __restrict__ off, const int* __restrict__ idx) {
+  const int px = blockIdx.x * 16 + threadIdx.x, py = blockIdx.y * 16 + threadIdx.y;
+  const int v = blockIdx.z;
+  const int ntx = (nx + 15) / 16, nty = (ny + 15) / 16, ntiles = ntx * nty;
+  const int t = blockIdx.y * ntx + blockIdx.x;
+  const int* o = off + (long long)v * (ntiles + 1);
+  const int a0 = o[t], a1 = o[t + 1];
+  if (px >= nx || py >= ny) return;
+  double pot = 0.0;
+  for (int k = a0; k < a1; ++k) {
+    const GaussAtom at = atoms[idx[k]];
+    // unwrapped rows j = py + m ny and columns i = px + l nx inside the range
+    int j = at.j0 + (((py - at.j0) % ny) + ny) % ny;
+    for (; j <= at.j1; j += ny) {
+      const double dy = j * hy - at.ya;
+      const double dy2 = dy * dy;
+      if (dy2 > rc2) continue;
+      int i = at.i0 + (((px - at.i0) % nx) + nx) % nx;
+      for (; i <= at.i1; i += nx) {
+        const double dx = i * hx - at.xa;
+        const double r2 = dx * dx + dy2;
+        if (r2 > rc2) continue;
+        pot += at.amp * exp(-r2 * inv2s2);
+      }
+    }
+  }
+  const double ph = sig_int * pot;
+  out[(long long)v * nx * ny + (long long)py * nx + px] = make_float2((float)cos(ph), (float)sin(ph));
+}
+
 // dst += w * src (complex), n items
 __global__ void k_axpy_c(float2* __restrict__ dst, const float2* __restrict__ src, long long n, float w) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -279,6 +320,15 @@ hipError_t launch_offset_wave(float2* out, const float2* in, int nx, int ny, flo
   const long long n = (long long)nx * ny;
   hipLaunchKernelGGL(k_offset_wave, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, in,
                      nx, ny, kscx, kscy, dx, dy, fz, cppl, ish0, ish1);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauss_gratings(float2* out, int nx, int ny, int nvar, double hx, double hy,
+                                 double rc2, double inv2s2, double sig_int, const GaussAtom* atoms,
+                                 const int* off, const int* idx, hipStream_t st) {
+  dim3 g((nx + 15) / 16, (ny + 15) / 16, nvar), b(16, 16);
+  hipLaunchKernelGGL(k_gauss_gratings, g, b, 0, st, out, nx, ny, hx, hy, rc2, inv2s2, sig_int, atoms, off,
+                     idx);
   return hipGetLastError();
 }
 

@@ -303,6 +303,21 @@ class Engine:
         self.sp = sp
         return self
 
+    def gaussian_gratings(self, islc, sites, sigma, sig_int):
+        """Generate the variants of structure slice islc on the device from
+        per-variant site lists [(xy [A] (k, 2), z (k,)), ...]."""
+        nat = _i32([len(z) for _, z in sites])
+        xy = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, 2) for x, _ in sites]),
+                                  np.float64)
+        z = _i32(np.concatenate([np.asarray(z) for _, z in sites]))
+        self._c("jms_hip_gaussian_gratings", islc, len(sites), ptr(nat), ptr(xy), ptr(z), float(sigma),
+                float(sig_int))
+
+    def gratings(self, islc, nvar) -> np.ndarray:
+        out = np.zeros((nvar, self.sp.ny, self.sp.nx), np.complex64)
+        self._c("jms_hip_get_gratings", islc, nvar, ptr(out))
+        return out
+
     def probe(self, alpha_mrad, alpha_rs=0.03, alpha_x0=0.0, alpha_y0=0.0, alpha_asym=0.0, alpha_adir=0.0,
               btx=0.0, bty=0.0, abrr=None) -> np.ndarray:
         """Generate the incident probe on the device (it becomes the scan's
