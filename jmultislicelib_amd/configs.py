@@ -53,6 +53,9 @@ class Setup:
     flags: int = 1                # detection channels
     slice_thickness: float = 0.0  # [nm]
     sites: list = field(default_factory=list)  # per (type, variant): (xy [A], z)
+    # per structure slice, per variant: (xy [A], z) atom sets whose gratings
+    # are generated on the device when `gratings` is None
+    grating_sites: list = None
 
 
 def detection_planes(ndetper: int, nobjslc: int) -> tuple[np.ndarray, int]:
@@ -135,6 +138,39 @@ def single_probe_256(**kw) -> Setup:
 def stem_scan_512(**kw) -> Setup:
     """512^2, 8x8 cells, 200 slices, 8 frozen-phonon variants per slice type."""
     return perovskite_setup(n=512, ncell=8, nslices=200, nvar=8, **kw)
+
+
+def thick_amorphous(n: int = 2048, cell_ang: float = 125.0, nslices: int = 1000, dz_ang: float = 2.0,
+                    density: float = 0.06, zs=(6, 8, 14), pz=(0.5, 0.3, 0.2), seed: int = 3,
+                    ht: float = 200.0, alpha: float = 25.0, alpha_rs: float = 0.03,
+                    det_ranges=DETECTOR_RANGES, flags: int = 1) -> Setup:
+    """Large thick specimen: amorphous random atoms (density [1/A^3], Z drawn
+    from zs with probabilities pz, uniform positions) in `nslices` distinct
+    slices of dz_ang over a cell_ang^2 cell, the Gaussian potential model;
+    the gratings are generated on the device (grating_sites)."""
+    rng = np.random.default_rng(seed)
+    a_nm = cell_ang * 0.1
+    dz_nm = dz_ang * 0.1
+    J = JMultiSlice()
+    J.SetHighTension(ht)
+    J.SetGridSize(n, n)
+    J.SetSupercellSize(a_nm, a_nm, dz_nm)
+    wl = J.GetWaveLength()
+    nat = int(round(density * cell_ang * cell_ang * dz_ang))
+    gsites = []
+    for _ in range(nslices):
+        xy = rng.uniform(0.0, cell_ang, (nat, 2))
+        z = rng.choice(np.asarray(zs), nat, p=np.asarray(pz)).astype(np.int32)
+        gsites.append([(xy, z)])
+    pro = J.CalculatePropagator(dz_nm, 0.0, 0.0, 1)
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in det_ranges]
+    probe = J.CalculateProbeWaveFourier(alpha, alpha_rs)
+    objslc = np.arange(nslices, dtype=np.int32)
+    det_objslc, ndetslc = detection_planes(0, nslices)
+    return Setup(nx=n, ny=n, a0=(a_nm, a_nm, dz_nm), ht=ht, wl=wl, nvarslc=[1] * nslices, gratings=None,
+                 propagators=[pro], proidx=[0] * nslices, objslc=objslc, det_objslc=det_objslc,
+                 ndetslc=ndetslc, detectors=dets, probe=probe, flags=flags, slice_thickness=dz_nm,
+                 grating_sites=gsites)
 
 
 def scan_positions(nscan: int = 64, ncells_scan: int = 2):

@@ -282,9 +282,15 @@ class Engine:
         self._c("jms_hip_set_grid", sp.nx, sp.ny, sp.a0[0], sp.a0[1], sp.wl)
         nv = _i32(sp.nvarslc)
         self._c("jms_hip_setup_gratings", sp.nx, sp.ny, len(nv), ptr(nv))
-        for i, g in enumerate(sp.gratings):
-            g = _c64(g)
-            self._c("jms_hip_upload_gratings", i, g.shape[0], ptr(g))
+        self.sp = sp
+        if sp.gratings is None:  # generated on the device from the atom sets
+            from .configs import SIG_INT, SIGMA_POT
+            for i, sets in enumerate(sp.grating_sites):
+                self.gaussian_gratings(i, sets, SIGMA_POT, SIG_INT)
+        else:
+            for i, g in enumerate(sp.gratings):
+                g = _c64(g)
+                self._c("jms_hip_upload_gratings", i, g.shape[0], ptr(g))
         pi = _i32(sp.proidx)
         self._c("jms_hip_setup_propagators", len(sp.propagators), len(pi), ptr(pi))
         for i, p in enumerate(sp.propagators):

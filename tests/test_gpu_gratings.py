@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from jmultislicelib_amd import Engine, configs
+from oracle import oracle as O
 
 from helpers import rel_l2
 
@@ -74,4 +75,30 @@ def test_multislice_on_device_gratings():
     e = rel_l2(got, ref)
     print(f"multislice on device vs host gratings: rel L2 {e:.2e} (other displacements: {rel_l2(other, ref):.2e})")
     assert e < 1e-5
+    eng.close()
+
+
+def test_thick_amorphous_small_vs_oracle():
+    """The large-specimen workload's recipe (distinct amorphous slices, device
+    gratings, no frozen phonons) at 256^2 x 24 slices: a scan of 5 probes
+    against the oracle run on host-generated gratings."""
+    sp = configs.thick_amorphous(n=256, cell_ang=15.625, nslices=24)
+    eng = Engine().setup(sp)
+    # host gratings for the oracle
+    host = []
+    for sets in sp.grating_sites:
+        host.append(np.stack([configs.gaussian_grating(sp.nx, 15.625, xy, z) for xy, z in sets]))
+    sp_h = configs.Setup(**{**sp.__dict__, "gratings": host})
+    dx = np.array([0.0, 0.3, 0.7, 1.1, 1.5], np.float32)
+    dy = np.array([0.0, 0.2, 0.9, 0.4, 1.3], np.float32)
+    var = np.zeros((len(dx), len(sp.objslc)), np.int32)
+    w = np.ones(1, np.float32)
+    eng.stage_scan(dx, dy, None, w, var)
+    eng.run_staged(0, len(dx))
+    eng.sync()
+    got = eng.scan_results()
+    ref = O.stem_scan(sp_h, dx, dy, w, var, prec=32, nthreads=4)
+    e = rel_l2(got, ref)
+    print(f"thick amorphous 256^2 x 24: rel L2 {e:.2e}")
+    assert e < 1e-4
     eng.close()
