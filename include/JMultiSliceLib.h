@@ -7,7 +7,7 @@
  *
  * Deliberate differences from the reference exports:
  *  - C types only: const char* instead of std::string, pointers instead of
- *    C++ references, no default arguments.
+ *    C++ references, int for bool, no default arguments.
  *  - CalculateRingDetector passes theta0y through (the reference wrapper passes
  *    theta0x twice, JMultiSliceLib.cpp:254).
  *  - OffsetIncomingWave takes bcppl and SetIncidentWave takes bTranspose (the
@@ -70,6 +70,13 @@ JMS_API float GetWaveLength(void);                                  /* :135 */
 JMS_API void SetSupercellSize(float* a0);                           /* :138 */
 JMS_API void SetSupercellSizeABC(float a, float b, float c);        /* :141 */
 JMS_API void SetSliceThickness(int islc, float fthickness);         /* :146 */
+JMS_API void DiffractionDescan(int bActivate);                      /* :149 */
+JMS_API void SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany,
+                                   int iThread);                      /* :160 */
+JMS_API void SetDiffractionDescan(int whichcode, float descanx, float descany,
+                                  int iThread);                       /* :168 */
+JMS_API void SetDiffractionDescanMRad(int whichcode, float descanx, float descany,
+                                      int iThread);                   /* :176 */
 
 /* numerics interface */
 JMS_API void SetGridSize(int nx, int ny);                           /* :183 */

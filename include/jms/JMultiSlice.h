@@ -8,8 +8,8 @@
 // Differences from the reference, by design:
 //  - The CPU code path (_JMS_CODE_CPU: CPUMultislice and the host multislice
 //    buffers) is not part of this library; requests for it return an error.
-//  - Plasmon Monte-Carlo, diffraction de-scan and sub-frame phase gratings
-//    are not implemented; setups that need them fail with an error.
+//  - Plasmon Monte-Carlo and sub-frame phase gratings are not implemented;
+//    setups that need them fail with an error.
 //  - The detection channels (integrated, image, diffraction, real- and
 //    Fourier-space waves and their averages) are filled by the single-wave
 //    path (GPUMultislice); the batched scan fills the integrated channel.
@@ -75,6 +75,11 @@ class CJMultiSlice {
   int SetDetectionPlanes(std::vector<int> slc_det, int nobjslc, int* det_objslc = nullptr);
   void SetPlasmonMC(bool use, float q_e, float q_c, float mfp, unsigned int nexmax);
   void DiffractionDescan(bool bActivate);
+  // de-scan of the Fourier-space readouts (JMultiSlice.h:471-495): in pixels,
+  // in 1/nm (rounded to pixels with the cell size) or in mrad
+  void SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany, int iThread = 0);
+  void SetDiffractionDescan(int whichcode, float descanx, float descany, int iThread = 0);
+  void SetDiffractionDescanMRad(int whichcode, float descanx, float descany, int iThread = 0);
 
   int CalculateProbeWaveFourier(CJProbeParams* prm, fcmplx* wav);
   int CalculatePropagator(float fthick, float otx, float oty, fcmplx* pro, int ntype = 0);
@@ -168,6 +173,8 @@ class CJMultiSlice {
   int m_ndet = 0, m_ndetper = 0, m_imagedet = 0, m_ndetslc = 0;
   int m_status_setup_GPU = 0;
   int m_plasmonmc_flg = 0, m_dif_descan_flg = 0;
+  int m_d_dif_ndescanx = 0, m_d_dif_ndescany = 0;  // de-scan [pixel]
+  int ApplyDescan();  // hands the active de-scan to the engine
   std::vector<int> m_objslc, m_det_objslc, m_nvarslc, m_proidx;
   std::vector<float> m_slcthick;
   std::vector<float> m_d_det_int;  // integrated GPU results live on the host (JMultiSlice.h:349-352)

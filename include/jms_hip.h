@@ -133,6 +133,11 @@ int jms_hip_wave_get(jms_hip_engine* e, float* dst);
  * run weights of the image group) or WAVER_AVG / WAVEF_AVG (2*ndetslc*nx*ny
  * floats, weight of the wave group) -- GetAvgResult :2489 */
 int jms_hip_get_avg_result(jms_hip_engine* e, int which, float* dst, float* weight);
+/* diffraction de-scan [pixel] of all Fourier-space readouts (integrated
+ * detectors, diffraction and Fourier-space wave channels): they see the wave
+ * shifted cyclically, psi'(i - d) = psi(i) (DescanDifN, JMultiSlice.cpp:
+ * 2184-2275); (0, 0) switches it off */
+int jms_hip_set_descan(jms_hip_engine* e, int dnx, int dny);
 /* summed run weights of the image and wave averaging groups -- :2567-2590 */
 int jms_hip_get_averaging_weights(jms_hip_engine* e, float* img_weight, float* wave_weight);
 /* group 0: image group (IMAGE_AVG, DIFFR_AVG), group 1: wave group (WAVER_AVG,

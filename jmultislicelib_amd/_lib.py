@@ -37,6 +37,10 @@ _SIGS = [
     ("SetSupercellSize", None, [c_void_p]),
     ("SetSupercellSizeABC", None, [c_float, c_float, c_float]),
     ("SetSliceThickness", None, [c_int, c_float]),
+    ("DiffractionDescan", None, [c_int]),
+    ("SetDiffractionDescanN", None, [c_int, c_int, c_int, c_int]),
+    ("SetDiffractionDescan", None, [c_int, c_float, c_float, c_int]),
+    ("SetDiffractionDescanMRad", None, [c_int, c_float, c_float, c_int]),
     ("SetGridSize", None, [c_int, c_int]),
     ("GetGridSize", None, [c_void_p, c_void_p]),
     ("GetDetetionSliceNum", c_int, []),
@@ -112,6 +116,7 @@ _SIGS = [
     ("jms_hip_get_result", c_int, [c_void_p, c_int, c_void_p]),
     ("jms_hip_get_avg_result", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("jms_hip_get_averaging_weights", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("jms_hip_set_descan", c_int, [c_void_p, c_int, c_int]),
     ("jms_hip_reset_averaging", c_int, [c_void_p, c_int]),
     ("jms_hip_readout_det_avg", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("jms_hip_fft2d", c_int, [c_int, c_int, c_int, c_void_p, c_int]),

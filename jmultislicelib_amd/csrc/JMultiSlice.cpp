@@ -127,11 +127,32 @@ void CJMultiSlice::SetPlasmonMC(bool use, float, float, float, unsigned int nexm
     std::cerr << "Warning: (SetPlasmonMC): plasmon Monte-Carlo is not implemented; multislice calls will fail."
               << std::endl;
 }
-void CJMultiSlice::DiffractionDescan(bool b) {
-  m_dif_descan_flg = b ? 1 : 0;
-  if (b)
-    std::cerr << "Warning: (DiffractionDescan): de-scan is not implemented; multislice calls will fail."
-              << std::endl;
+void CJMultiSlice::DiffractionDescan(bool b) { m_dif_descan_flg = b ? 1 : 0; }
+
+// SetDiffractionDescanN / SetDiffractionDescan / SetDiffractionDescanMRad
+// (JMultiSlice.cpp:1118-1164), GPU branch: the pixel shift takes effect in the
+// next multislice run while the de-scan flag is on
+void CJMultiSlice::SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany, int iThread) {
+  (void)iThread;
+  if (cpu_code(whichcode) || !gpu_code(whichcode)) return;
+  if ((m_status_setup_GPU & _JMS_THRESHOLD_CALC) == 0) return;
+  m_d_dif_ndescanx = ndescanx;
+  m_d_dif_ndescany = ndescany;
+}
+
+void CJMultiSlice::SetDiffractionDescan(int whichcode, float descanx, float descany, int iThread) {
+  SetDiffractionDescanN(whichcode, (int)std::round(descanx * m_a0[0]), (int)std::round(descany * m_a0[1]),
+                        iThread);  // 1/nm -> pixel
+}
+
+void CJMultiSlice::SetDiffractionDescanMRad(int whichcode, float descanx, float descany, int iThread) {
+  SetDiffractionDescan(whichcode, descanx * 0.001f / m_wl, descany * 0.001f / m_wl, iThread);  // mrad -> 1/nm
+}
+
+int CJMultiSlice::ApplyDescan() {
+  const int dx = m_dif_descan_flg ? m_d_dif_ndescanx : 0;
+  const int dy = m_dif_descan_flg ? m_d_dif_ndescany : 0;
+  return jms_hip_set_descan(m_eng, dx, dy);
 }
 
 // JMultiSlice.cpp:579-627: plane -> detection index hash, returns #planes
@@ -805,10 +826,11 @@ int CJMultiSlice::SetIncomingWaveGPU(fcmplx* wav, bool bTranspose) {
 int CJMultiSlice::GPUMultislice(int islc0, int accmode, float weight) {
   if (!m_eng || !m_wave_ready || m_nscx * m_nscy <= 0) return 1;
   if ((m_status_setup_GPU & _JMS_THRESHOLD_CALC) != _JMS_THRESHOLD_CALC) return 2;
-  if (m_plasmonmc_flg || m_dif_descan_flg) {
-    std::cerr << "Error: (GPUMultislice): plasmon Monte-Carlo and de-scan are not implemented." << std::endl;
+  if (m_plasmonmc_flg) {
+    std::cerr << "Error: (GPUMultislice): plasmon Monte-Carlo is not implemented." << std::endl;
     return 5;
   }
+  if (ApplyDescan() != 0) return 6;
   std::vector<int> var((size_t)std::max(1, m_nobjslc), 0);
   const int i0 = std::max(0, std::min(islc0, m_nobjslc));
   if (i0 < m_nobjslc) GetRandomVariantSequence(var.data());  // drawn for the whole stack (:4582)
@@ -885,7 +907,8 @@ int CJMultiSlice::GPUMultisliceScanVar(int npos, const float* dx, const float* d
                                        const float* weights, const int* var, float* out) {
   if (!m_eng || npos < 1 || npass < 1 || !dx || !dy || !weights || !out) return 1;
   if ((m_status_setup_GPU & _JMS_THRESHOLD_CALC) != _JMS_THRESHOLD_CALC) return 2;
-  if (m_plasmonmc_flg || m_dif_descan_flg) return 5;
+  if (m_plasmonmc_flg) return 5;
+  if (ApplyDescan() != 0) return 6;
   if (jms_hip_stage_scan(m_eng, npos, dx, dy, dz, npass, weights, var) != 0) return 101;
   if (jms_hip_run_staged(m_eng, 0, npos, 0) != 0) return 102;
   if (jms_hip_get_scan_results(m_eng, out) != 0) return 103;

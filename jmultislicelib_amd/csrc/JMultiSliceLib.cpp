@@ -27,6 +27,16 @@ float GetWaveLength(void) { return jms_obj().GetWaveLength(); }
 void SetSupercellSize(float* a0) { jms_obj().SetSupercellSize(a0); }
 void SetSupercellSizeABC(float a, float b, float c) { jms_obj().SetSupercellSize(a, b, c); }
 void SetSliceThickness(int islc, float t) { jms_obj().SetSliceThickness(islc, t); }
+void DiffractionDescan(int bActivate) { jms_obj().DiffractionDescan(bActivate != 0); }
+void SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany, int iThread) {
+  jms_obj().SetDiffractionDescanN(whichcode, ndescanx, ndescany, iThread);
+}
+void SetDiffractionDescan(int whichcode, float descanx, float descany, int iThread) {
+  jms_obj().SetDiffractionDescan(whichcode, descanx, descany, iThread);
+}
+void SetDiffractionDescanMRad(int whichcode, float descanx, float descany, int iThread) {
+  jms_obj().SetDiffractionDescanMRad(whichcode, descanx, descany, iThread);
+}
 
 void SetGridSize(int nx, int ny) { jms_obj().SetGridSize(nx, ny); }
 void GetGridSize(int* nx, int* ny) {

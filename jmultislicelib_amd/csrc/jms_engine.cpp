@@ -93,6 +93,8 @@ struct jms_hip_engine {
   // reset, with the summed run weights of the image and wave groups
   DevBuf dif_avg, wff_avg, img_avg, wfr_avg;
   float w_imgavg = 0.f, w_wavavg = 0.f;
+  // diffraction de-scan [pixel] of the Fourier-space readouts
+  int dsx = 0, dsy = 0;
   // graph cache: key = (nwave, islc0)
   bool use_graph = false;
   std::map<std::pair<int, int>, hipGraphExec_t> graphs;
@@ -231,6 +233,8 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   ca.wff = ch.wff;
   ca.dif_avg = ch.dif_avg;
   ca.wff_avg = ch.wff_avg;
+  ca.dsx = e->dsx;
+  ca.dsy = e->dsy;
   ca.cwgt = ch.cwgt;
   ca.chan_wstride = (long long)e->ndetslc * e->n;
   jms::RowChanHost rc;
@@ -997,6 +1001,14 @@ int jms_hip_get_avg_result(jms_hip_engine* e, int which, float* dst, float* weig
   err = hipMemcpy(dst, b->p, nb, hipMemcpyDeviceToHost);
   if (err != hipSuccess) return e->fail(102, "read averaging channel", err);
   if (weight) *weight = w;
+  return 0;
+}
+
+int jms_hip_set_descan(jms_hip_engine* e, int dnx, int dny) {
+  if (!e) return 1;
+  if (e->dsx != dnx || e->dsy != dny) e->drop_graphs();
+  e->dsx = dnx;
+  e->dsy = dny;
   return 0;
 }
 

@@ -25,6 +25,7 @@ struct ColArgsHost {
   float* dif = nullptr;          // diffraction channel |psi|^2 (or nullptr)
   void* wff = nullptr;           // Fourier-space wave channel (float2, or nullptr)
   float* dif_avg = nullptr;      // averaged diffraction channel (or nullptr)
+  int dsx = 0, dsy = 0;          // diffraction de-scan [pixel]
   void* wff_avg = nullptr;       // averaged Fourier-space wave channel (or nullptr)
   const float* cwgt = nullptr;   // per batch entry weight of the channels
   long long chan_wstride = 0;    // per batch entry stride of the channels (items)

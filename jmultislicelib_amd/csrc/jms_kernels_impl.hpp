@@ -53,6 +53,12 @@ struct ColLayout : LdsPos<SW, PADSH> {
   static constexpr int words(int lines) { return lines * STRIDE; }
 };
 
+// index of i - d on an n-point periodic axis (0 <= i < n, any d)
+__device__ __forceinline__ int wrap_sub(int i, int d, int n) {
+  int r = (i - d) % n;
+  return r < 0 ? r + n : r;
+}
+
 // frequency index of position i on an n-point grid in FFT order
 // (JMultiSlice.cpp:1699-1712)
 __device__ __forceinline__ int fidx(int i, int n) {
@@ -212,6 +218,7 @@ __device__ __forceinline__ void block_reduce_det(float (&acc)[kMaxDet], int ndet
 // grid: (ceil(nx/C), nwave); block: TC*C
 // ---------------------------------------------------------------------------
 struct ColArgs {
+  int dsx, dsy;             // diffraction de-scan [pixel]: readouts see psi(i + d)
   float* dif_avg;           // averaged diffraction channel (or nullptr)
   float2* wff_avg;          // averaged Fourier-space wave channel (or nullptr)
   float2* wave;
@@ -329,15 +336,19 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
     for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
     if (valid) {
       const long long n = (long long)N * nx;
+      // de-scan (DescanDifN, JMultiSlice.cpp:2184-2275): the detectors integrate
+      // psi(i + d), i.e. pixel i is weighted with det(i - d)
+      const int xs = wrap_sub(x, a.dsx, nx);
       for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
-        const float* dw = a.detw + d * n + x;
+        const float* dw = a.detw + d * n + xs;
         float wt[NREG];
         static_for<0, M1::NB>([&](auto I) {  // all weight loads of this detector first
           constexpr int i = decltype(I)::value;
           const int b = t + i * T;
           static_for<0, M1::R0>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF) ? dw[(long long)(b + q * M1::NBF) * nx] : 0.f;
+            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF)
+                                     ? dw[(long long)wrap_sub(b + q * M1::NBF, a.dsy, N) * nx] : 0.f;
           });
         });
         float sacc = 0.f;
@@ -369,7 +380,7 @@ __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
         if (M1::FULL || b < M1::NBF) {
           static_for<0, M1::R0>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
+            const long long idx = cb + (long long)wrap_sub(b + q * M1::NBF, a.dsy, N) * nx + wrap_sub(x, a.dsx, nx);
             const float2 z = v[i * M1::R0 + q];
             const float pz = z.x * z.x + z.y * z.y;
             if (a.dif) a.dif[idx] += wg * pz;
@@ -554,15 +565,18 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
           pw[i * M1::R0 + q] = z.x * z.x + z.y * z.y;
         });
       });
+      // de-scan (DescanDifN, JMultiSlice.cpp:2184-2275): pixel i is weighted
+      // with det(i - d)
+      const long long colbs = (long long)wrap_sub(x, a.dsx, nx) * N;
       for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
-        const float* dw = a.detw_t + d * n + colb;
+        const float* dw = a.detw_t + d * n + colbs;
         float wt[NREG];
         static_for<0, M1::NB>([&](auto I) {  // all weight loads of this detector first
           constexpr int i = decltype(I)::value;
           const int b = t + i * T;
           static_for<0, M1::R0>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF) ? dw[b + q * M1::NBF] : 0.f;
+            wt[i * M1::R0 + q] = (M1::FULL || b < M1::NBF) ? dw[wrap_sub(b + q * M1::NBF, a.dsy, N)] : 0.f;
           });
         });
         float sacc = 0.f;
@@ -592,7 +606,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
         if (M1::FULL || b < M1::NBF) {
           static_for<0, M1::R0>([&](auto Q) {
             constexpr int q = decltype(Q)::value;
-            const long long idx = cb + (long long)(b + q * M1::NBF) * nx + x;
+            const long long idx = cb + (long long)wrap_sub(b + q * M1::NBF, a.dsy, N) * nx + wrap_sub(x, a.dsx, nx);
             const float2 z = v[i * M1::R0 + q];
             const float pz = z.x * z.x + z.y * z.y;
             if (a.dif) a.dif[idx] += wg * pz;
@@ -830,6 +844,7 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   a.part_wstride = h.part_wstride; a.part_dstride = h.part_dstride; a.part_off = h.part_off;
   a.dif = h.dif; a.wff = (float2*)h.wff; a.cwgt = h.cwgt; a.chan_wstride = h.chan_wstride;
   a.dif_avg = h.dif_avg; a.wff_avg = (float2*)h.wff_avg;
+  a.dsx = h.dsx; a.dsy = h.dsy;
   a.chan_off = h.chan_off;
   a.twg = (const float2*)h.twy;
   a.pro_t = (const float2*)h.pro_t;

@@ -81,6 +81,18 @@ class JMultiSlice:
     def SeedRngEx(self, seed: int) -> None:
         self.L.SeedRngEx(seed)
 
+    def DiffractionDescan(self, on: bool) -> None:
+        self.L.DiffractionDescan(int(bool(on)))
+
+    def SetDiffractionDescanN(self, whichcode, ndescanx, ndescany) -> None:
+        self.L.SetDiffractionDescanN(whichcode, int(ndescanx), int(ndescany), 0)
+
+    def SetDiffractionDescan(self, whichcode, descanx, descany) -> None:
+        self.L.SetDiffractionDescan(whichcode, descanx, descany, 0)
+
+    def SetDiffractionDescanMRad(self, whichcode, descanx, descany) -> None:
+        self.L.SetDiffractionDescanMRad(whichcode, descanx, descany, 0)
+
     def GetDetetionSliceNum(self) -> int:
         return self.L.GetDetetionSliceNum()
 
@@ -402,6 +414,9 @@ class Engine:
         w = C.c_float(0.0)
         self._c("jms_hip_get_avg_result", which, ptr(out), C.byref(w))
         return out, w.value
+
+    def set_descan(self, dnx: int, dny: int):
+        self._c("jms_hip_set_descan", int(dnx), int(dny))
 
     def averaging_weights(self):
         wi, ww = C.c_float(0.0), C.c_float(0.0)

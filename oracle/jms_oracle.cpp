@@ -907,6 +907,7 @@ struct OrcSetup {
   const int* msk;          // [ndet][n] masks (may be NULL)
   const int* msklen;       // [ndet] mask lengths (may be NULL)
   int imagedet;            // flags: 1 integrated, 16 wave Fourier
+  int dsx = 0, dsy = 0;    // diffraction de-scan [pixel] (0, 0: off)
 };
 
 // detection flags (JMultiSlice.h:141-150); the _AVG channels accumulate the
@@ -947,13 +948,32 @@ static void readout_img(const OrcSetup& S, const std::complex<T>* wav, int iplan
   }
 }
 
+// DescanDifN / DescanDifWavN_h (JMultiSlice.cpp:2184-2320): the de-scanned
+// copy of a Fourier-space array, out[(i - dnx) mod nx + ((j - dny) mod ny) nx]
+// = in[i + j nx]
+template <typename V>
+static void descan_copy(const V* in, V* out, int nx, int ny, int dnx, int dny) {
+  for (int j = 0; j < ny; j++) {
+    const int j1 = ((j - dny) % ny + ny) % ny;
+    for (int i = 0; i < nx; i++) out[((i - dnx) % nx + nx) % nx + (size_t)j1 * nx] = in[i + (size_t)j * nx];
+  }
+}
+
 template <typename T>
-static void readout_dif(const OrcSetup& S, const std::complex<T>* wav, int iplane, float weight,
+static void readout_dif(const OrcSetup& S, const std::complex<T>* wav_in, int iplane, float weight,
                         float* out_int, T* out_wff, std::vector<float>& difbuf,
                         std::vector<T>& difbuf_d, const OrcChan<T>& ch = OrcChan<T>()) {
   const int n = S.nx * S.ny;
   const int idetslc = S.det_objslc[iplane];
   if (idetslc < 0) return;
+  // with de-scan, every Fourier-space readout sees the shifted wave
+  std::vector<std::complex<T>> shifted;
+  const std::complex<T>* wav = wav_in;
+  if (S.dsx != 0 || S.dsy != 0) {
+    shifted.resize(n);
+    descan_copy(wav_in, shifted.data(), S.nx, S.ny, S.dsx, S.dsy);
+    wav = shifted.data();
+  }
   if ((S.imagedet & kDetDif) && ch.dif) {  // ReadoutDifDet_h (:3369-3470): out += weight * |psi|^2
     T* o = ch.dif + (size_t)n * idetslc;
     for (int i = 0; i < n; ++i) {
@@ -1076,17 +1096,20 @@ ORC_API int orc_multislice(int prec, int nx, int ny, int nscslc, const int* nvar
 
 // orc_multislice with the intensity and wave channels: out_dif and out_img
 // (n*ndetslc items), out_wfr and out_wff (2*n*ndetslc items), each NULL or
-// accumulated with `weight`; float for prec 32, double for prec 64.
+// accumulated with `weight`; float for prec 32, double for prec 64.  dsx, dsy:
+// diffraction de-scan [pixel] applied to the Fourier-space readouts.
 ORC_API int orc_multislice_ch(int prec, int nx, int ny, int nscslc, const int* nvarslc,
                               const long long* slcoffset, const float* pgr, int npro,
                               const int* proidx, const float* pro, int nobjslc, const int* objslc,
                               const int* det_objslc, int ndetslc, int ndet, const float* det,
                               const int* msk, const int* msklen, int imagedet, const float* wave_in,
                               const int* var, int islc0, float weight, float* out_int, void* out_dif,
-                              void* out_img, void* out_wfr, void* out_wff) {
+                              void* out_img, void* out_wfr, void* out_wff, int dsx, int dsy) {
   if (nx < 1 || ny < 1 || !wave_in || !det_objslc) return 1;
   OrcSetup S = make_setup(nx, ny, nscslc, nvarslc, slcoffset, pgr, npro, proidx, pro, nobjslc,
                           objslc, det_objslc, ndetslc, ndet, det, msk, msklen, imagedet);
+  S.dsx = dsx;
+  S.dsy = dsy;
   const int n = nx * ny;
   std::vector<float> difbuf(n);
   if (prec == 64) {

@@ -44,7 +44,7 @@ _SIGS = {
     "orc_multislice": (i_, [i_, i_, i_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_, p_, i_, p_,
                             p_, i_, f_, p_, p_]),
     "orc_multislice_ch": (i_, [i_, i_, i_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_, p_, i_, p_,
-                               p_, i_, f_, p_, p_, p_, p_, p_]),
+                               p_, i_, f_, p_, p_, p_, p_, p_, i_, i_]),
     "orc_stem_scan": (i_, [i_, i_, i_, i_, f_, f_, f_, i_, p_, p_, p_, i_, p_, p_, i_, p_, p_, i_, i_, p_, p_,
                            p_, i_, p_, p_, i_, p_, p_, p_, i_, p_, p_, p_, p_]),
     "orc_perovskite_sites": (i_, [i_, i_, d_, i_, i_, i_, i_, d_, i_, i_, p_, p_]),
@@ -251,11 +251,12 @@ def multislice(sp, wave_in, var, weight=1.0, prec=32, islc0=0, want_wave=False, 
 
 
 def multislice_channels(sp, wave_in, var, weight=1.0, prec=32, islc0=0, flags=1 | 4 | 2 | 8 | 16,
-                        chans=None):
+                        chans=None, descan=(0, 0)):
     """One CPUMultislice pass with the channels selected by `flags`
     (1 integrated, 2 image, 4 diffraction, 8 real-space wave, 16 Fourier-space
     wave; ReadoutDifDet_h / ReadoutImgDet_h, JMultiSlice.cpp:3369-3538).
-    Accumulates into `chans` (a dict from a previous call) when given.
+    Accumulates into `chans` (a dict from a previous call) when given;
+    descan: (dnx, dny) pixel de-scan of the Fourier-space readouts.
     Returns dict: int [ndetslc, ndet], dif/img [ndetslc, ny, nx],
     wfr/wff [ndetslc, ny, nx] complex."""
     P = _Packed(sp)
@@ -275,7 +276,8 @@ def multislice_channels(sp, wave_in, var, weight=1.0, prec=32, islc0=0, flags=1 
                                len(sp.propagators), _p(P.proidx), _p(P.pro), len(sp.objslc), _p(P.objslc),
                                _p(P.det_objslc), sp.ndetslc, nd, _p(P.det), _p(P.msk), _p(P.msklen), flags,
                                _p(w), _p(v), islc0, weight, _p(chans["int"]), _p(chans["dif"]),
-                               _p(chans["img"]), _p(chans["wfr"]), _p(chans["wff"]))
+                               _p(chans["img"]), _p(chans["wfr"]), _p(chans["wff"]), int(descan[0]),
+                               int(descan[1]))
     assert rc == 0
     return chans
 

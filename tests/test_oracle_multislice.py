@@ -177,3 +177,24 @@ def test_oracle_channels_vs_numpy(nx, ny, ndetper):
     # accumulation: a second call adds the same amounts
     again = O.multislice_channels(sp, win, var, 0.7, prec=64, chans=got64)
     assert np.allclose(again["img"], 2 * ref["img"], rtol=1e-10, atol=1e-12 * np.abs(ref["img"]).max())
+
+
+def test_oracle_descan():
+    """De-scan (DescanDifN, JMultiSlice.cpp:2184-2275) shifts the
+    Fourier-space readouts cyclically: out(i - d) = plain(i); the real-space
+    channels are unchanged, and the integrated detector sums equal the plain
+    pattern integrated with the detector shifted by +d."""
+    sp = random_setup(32, 32, nslices=4, nvar=(2, 2), ndetper=2, seed=12, det_ranges=((0, 12), (12, 40)))
+    var = configs.variant_sequences(sp, 1, seed=1)[0]
+    win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.02, 0.04)
+    d = (3, -2)
+    a = O.multislice_channels(sp, win, var, 1.0, prec=64)
+    b = O.multislice_channels(sp, win, var, 1.0, prec=64, descan=d)
+    for k in ("dif", "wff"):
+        assert np.array_equal(b[k], np.roll(np.roll(a[k], -d[1], axis=1), -d[0], axis=2)), k
+    for k in ("img", "wfr"):
+        assert np.array_equal(b[k], a[k]), k
+    for di, (det, _) in enumerate(sp.detectors):
+        shifted = np.roll(np.roll(det.astype(np.float64), d[1], axis=0), d[0], axis=1)
+        ref = np.einsum("pyx,yx->p", a["dif"], shifted)
+        assert np.allclose(b["int"][:, di], ref, rtol=2e-7), di
