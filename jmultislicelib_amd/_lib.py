@@ -125,6 +125,12 @@ _SIGS = [
     ("jms_hip_last_error", c_char_p, [c_void_p]),
     # variant sequencing (include/jms_sequence.h)
     ("jms_variant_sequences", c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    # on-disk formats (include/jms_io.h)
+    ("jms_ems_read_header", c_int, [c_char_p, c_void_p]),
+    ("jms_ems_read_data", c_int, [c_char_p, c_void_p, c_void_p]),
+    ("jms_ems_write", c_int, [c_char_p, c_void_p, c_void_p]),
+    ("jms_stem_images", c_int, [c_int, c_int, c_int, c_void_p, c_float, c_void_p]),
+    ("jms_bin_write_channel", c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
     # specimen builder (include/jms_specimen.h)
     ("jms_perovskite_sites", c_int, [c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_double, c_int,
                                      c_int, c_void_p, c_void_p]),

@@ -189,3 +189,51 @@ def variant_sequences(sp: Setup, nseq: int, seed: int = 42) -> np.ndarray:
                                      ptr(out))
     assert rc == 0
     return out
+
+
+def slice_file_name(prefix: str, islc: int, suffix: str = ".sli") -> str:
+    """Name of structure slice islc (0-based) in a numbered series:
+    prefix_001.sli, prefix_002.sli, ... (params.cpp:110-115,
+    prm_sample.cpp:599-615)."""
+    return f"{prefix}_{islc + 1:03d}{suffix}"
+
+
+def setup_from_slice_files(paths, nobj: int, ndetper: int = 0, alpha: float = 25.0, alpha_rs: float = 0.03,
+                           tilt_mrad=(0.0, 0.0), ptype: int = 1, det_ranges=DETECTOR_RANGES,
+                           flags: int = 1) -> Setup:
+    """Set-up from EMS/SLI phase-grating files, one per structure slice, as
+    the reference bench prepares it: grid, super-cell and high tension from
+    the first file's header, one propagator per structure slice from its own
+    header thickness (prm_slice.cpp:495-521, prm_sample.cpp:1231-1282) with
+    the tilt given in mrad, periodic stacking of nobj object slices
+    (prm_sample.cpp:283-300) and detection planes every ndetper slices."""
+    from .io import read_ems
+    if not paths:
+        raise ValueError("no slice files")
+    heads, gratings = [], []
+    for p in paths:
+        h, d = read_ems(p)
+        heads.append(h)
+        gratings.append(d)
+    h0 = heads[0]
+    nx, ny = int(h0.grid_x), int(h0.grid_y)
+    for p, h in zip(paths, heads):
+        if (h.grid_x, h.grid_y) != (nx, ny):
+            raise ValueError(f"{p}: grid {h.grid_x}x{h.grid_y} differs from {nx}x{ny}")
+    J = JMultiSlice()
+    J.SetHighTension(float(h0.ekv))
+    J.SetGridSize(nx, ny)
+    J.SetSupercellSize(float(h0.sx), float(h0.sy), float(h0.sz))
+    wl = J.GetWaveLength()
+    ttf = np.float32(180.0 / np.pi) * np.float32(0.001)  # mrad -> degree
+    otx, oty = float(ttf * np.float32(tilt_mrad[0])), float(ttf * np.float32(tilt_mrad[1]))
+    pros = [J.CalculatePropagator(float(h.sz), otx, oty, ptype) for h in heads]
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in det_ranges]
+    probe = J.CalculateProbeWaveFourier(alpha, alpha_rs)
+    nslc = len(paths)
+    objslc = np.array([i % nslc for i in range(nobj)], np.int32)
+    det_objslc, ndetslc = detection_planes(ndetper, nobj)
+    return Setup(nx=nx, ny=ny, a0=(float(h0.sx), float(h0.sy), float(h0.sz)), ht=float(h0.ekv), wl=wl,
+                 nvarslc=[int(h.num_var) for h in heads], gratings=gratings, propagators=pros,
+                 proidx=list(range(nslc)), objslc=objslc, det_objslc=det_objslc, ndetslc=ndetslc,
+                 detectors=dets, probe=probe, flags=flags, slice_thickness=float(h0.sz))

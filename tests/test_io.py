@@ -1,0 +1,151 @@
+"""EMS/SLI slice files and .bin result channels (include/jms_io.h) against the
+format as the reference's JMSBench1 defines it (prm_slice_file_info.h:37-55,
+63-143; prm_slice_file_info.cpp:79-196; prm_slice.cpp:253-300;
+prm_result.cpp:192-290; multislice.cpp:901-916).  The reference repository
+holds no slice files (its .sli inputs are not committed), so files are built
+here byte by byte from the documented offsets, independently of the
+library's writer."""
+import numpy as np
+import pytest
+
+from jmultislicelib_amd import io as jio
+
+OFF = dict(nx=0x0, ny=0x4, title=0x4C, ver=0x74, nvar=0x78, ctype=0x80, alt=0x88, itab=0x8C,
+           thick=0x1034, ekv=0x1040, sx=0x1044, sy=0x1048)
+
+
+def _build(path, data, ver, endian="<", alt=None, itab=None, title=b"slice", sx=1.5, sy=1.25, sz=0.2, ekv=300.0):
+    nv, ny, nx = data.shape
+    doff = alt if alt is not None else 0x2000
+    buf = bytearray(doff)
+    i4, f4 = np.dtype(endian + "i4"), np.dtype(endian + "f4")
+
+    def put(pos, val, dt):
+        buf[pos:pos + 4] = np.array([val], dt).tobytes()
+    put(OFF["nx"], nx, i4)
+    put(OFF["ny"], ny, i4)
+    buf[OFF["title"]:OFF["title"] + len(title)] = title
+    put(OFF["ver"], ver, np.dtype(endian + "u4"))
+    put(OFF["nvar"], nv, i4)
+    put(OFF["ctype"], 0, i4)
+    if alt is not None:
+        put(OFF["alt"], alt, i4)
+        put(OFF["itab"], itab, i4)
+    put(OFF["thick"], sz, f4)
+    put(OFF["ekv"], ekv, f4)
+    put(OFF["sx"], sx, f4)
+    put(OFF["sy"], sy, f4)
+    with open(path, "wb") as f:
+        f.write(bytes(buf))
+        f.write(np.ascontiguousarray(data, np.complex64).tobytes())  # data are never swapped
+
+
+def _data(nv, ny, nx, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.exp(1j * rng.uniform(-3, 3, (nv, ny, nx))).astype(np.complex64)
+
+
+@pytest.mark.parametrize("ver,alt", [(2020052801, 0x2000), (2012071101, 0x3000), (2010112301, None)])
+def test_read_documented_layout(tmp_path, ver, alt):
+    d = _data(3, 16, 24)
+    p = str(tmp_path / "a.sli")
+    _build(p, d, ver, alt=alt, itab=0x1100 if alt else None)
+    h, got = jio.read_ems(p)
+    assert (h.grid_x, h.grid_y, h.num_var, h.version, h.byte_swapped) == (24, 16, 3, ver, 0)
+    assert h.data_offset == (alt or 0x2000)
+    assert (h.sx, h.sy, h.ekv) == (np.float32(1.5), np.float32(1.25), np.float32(300.0))
+    assert abs(h.sz - 0.2) < 1e-7 and h.title == b"slice"
+    assert np.array_equal(got, d)
+
+
+def test_unknown_version_means_one_variant(tmp_path):
+    d = _data(2, 8, 8)
+    p = str(tmp_path / "b.sli")
+    _build(p, d, 12345)
+    h, got = jio.read_ems(p)
+    assert h.num_var == 1 and np.array_equal(got[0], d[0])
+
+
+def test_byte_swapped_header(tmp_path):
+    """Big-endian header: detected by the first int, fields swapped; the data
+    block is read as stored (the reference reads it with item size 1)."""
+    d = _data(2, 8, 16, seed=4)
+    p = str(tmp_path / "c.sli")
+    _build(p, d, 2020052801, endian=">", alt=0x2000, itab=0x1100)
+    h, got = jio.read_ems(p)
+    assert h.byte_swapped == 1 and (h.grid_x, h.grid_y, h.num_var) == (16, 8, 2)
+    assert h.ekv == np.float32(300.0)
+    assert np.array_equal(got, d)
+
+
+def test_writer_round_trip_and_offsets(tmp_path):
+    d = _data(4, 32, 48, seed=7)
+    p = str(tmp_path / "w.sli")
+    jio.write_ems(p, d, 1.2, 0.9, 0.19525, 200.0, title="STO z=0")
+    raw = open(p, "rb").read()
+    i4 = lambda o: int(np.frombuffer(raw[o:o + 4], "<i4")[0])
+    f4 = lambda o: float(np.frombuffer(raw[o:o + 4], "<f4")[0])
+    assert (i4(OFF["nx"]), i4(OFF["ny"]), i4(OFF["nvar"])) == (48, 32, 4)
+    assert i4(OFF["ver"]) == 2020052801 and i4(OFF["alt"]) == 0x2000 and i4(OFF["itab"]) == 0x1100
+    assert f4(OFF["ekv"]) == 200.0 and abs(f4(OFF["thick"]) - 0.19525) < 1e-7
+    assert raw[OFF["title"]:OFF["title"] + 7] == b"STO z=0"
+    assert len(raw) == 0x2000 + d.nbytes
+    assert np.array_equal(np.frombuffer(raw[0x2000:], np.complex64).reshape(d.shape), d)
+    h, got = jio.read_ems(p)
+    assert np.array_equal(got, d) and abs(h.sx - 1.2) < 1e-7 and abs(h.sy - 0.9) < 1e-7
+
+
+def test_stem_images_and_bin(tmp_path):
+    """Result layout of the bench: out[pos + npos*(plane + nplanes*det)] =
+    res[pos][plane][det] / weight, and one channel per .bin file."""
+    rng = np.random.default_rng(1)
+    res = rng.random((12, 3, 4)).astype(np.float32)
+    im = jio.stem_images(res, 0.5)
+    flat = im.reshape(-1)
+    for p in range(12):
+        for i in range(3):
+            for j in range(4):
+                assert flat[p + 12 * i + 36 * j] == np.float32(res[p, i, j] / np.float32(0.5))
+    f = str(tmp_path / "ch2.bin")
+    jio.write_bin_channel(f, im, 2)
+    assert np.array_equal(np.fromfile(f, np.float32), im[2].reshape(-1))
+
+
+def _write_series(tmp_path, sp, prefix="pv"):
+    from jmultislicelib_amd import configs
+    paths = []
+    for i, g in enumerate(sp.gratings):
+        p = str(tmp_path / configs.slice_file_name(prefix, i))
+        jio.write_ems(p, g, sp.a0[0], sp.a0[1], sp.slice_thickness, sp.ht, title=f"slice {i}")
+        paths.append(p)
+    return paths
+
+
+def test_setup_from_slice_files(tmp_path):
+    """A slice series written from a built-in specimen loads back into the
+    same set-up: identical gratings, propagators, probe and detectors,
+    periodic stacking of the object slices."""
+    from jmultislicelib_amd import configs
+    sp = configs.perovskite_setup(n=32, ncell=2, nslices=11, nvar=3, ndetper=4)
+    paths = _write_series(tmp_path, sp)
+    assert paths[1].endswith("pv_002.sli")
+    s2 = configs.setup_from_slice_files(paths, 11, ndetper=4)
+    assert (s2.nx, s2.ny, s2.nvarslc, s2.wl) == (sp.nx, sp.ny, sp.nvarslc, sp.wl)
+    assert np.array_equal(s2.objslc, sp.objslc) and np.array_equal(s2.det_objslc, sp.det_objslc)
+    assert all(np.array_equal(a, b) for a, b in zip(s2.gratings, sp.gratings))
+    for k in s2.proidx:
+        assert np.array_equal(s2.propagators[k].view(np.uint32), sp.propagators[0].view(np.uint32))
+    assert np.array_equal(s2.probe, sp.probe)
+    for (a, ma), (b, mb) in zip(s2.detectors, sp.detectors):
+        assert np.array_equal(a, b) and np.array_equal(ma, mb)
+
+
+def test_setup_from_slice_files_rejects_mixed_grids(tmp_path):
+    from jmultislicelib_amd import configs
+    d = _data(1, 16, 16)
+    jio.write_ems(str(tmp_path / "a_001.sli"), d, 1.0, 1.0, 0.2, 200.0)
+    jio.write_ems(str(tmp_path / "a_002.sli"), _data(1, 16, 32), 1.0, 1.0, 0.2, 200.0)
+    with pytest.raises(ValueError):
+        configs.setup_from_slice_files([str(tmp_path / "a_001.sli"), str(tmp_path / "a_002.sli")], 4)
+    with pytest.raises(RuntimeError):
+        jio.read_ems(str(tmp_path / "missing.sli"))
