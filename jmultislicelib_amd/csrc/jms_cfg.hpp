@@ -36,8 +36,12 @@ struct Cfg {
   static constexpr int TW = pow2_floor(T < 64 ? T : 64);
   static constexpr int CW = 16;
   static constexpr bool WL = (N <= 768) && ((TW * CW) % 64 == 0);
+  // long columns (N = 1024, 2048): register-resident column kernel with 16
+  // columns per block (128-byte row segments; a 2048^2 column-tile copy runs
+  // at 0.93 / 2.6 / 5.1 TB/s with 4 / 8 / 16 columns per block on MI355X)
+  static constexpr bool XL = (N == 1024 || N == 2048);
   // columns per block of the column kernel in use (partial-sum tiles)
-  static constexpr int CC = WL ? CW : C;
+  static constexpr int CC = XL ? 16 : (WL ? CW : C);
   static constexpr int PADSH = 3;
   // minimum waves per SIMD requested from the register allocator for the
   // wave-local column kernel (1 = no constraint).  At N = 512 eight waves let

@@ -282,6 +282,25 @@ template <int DIR> struct DFT<12, DIR> : DFTComp<4, 3, DIR> {};
 template <int DIR> struct DFT<14, DIR> : DFTComp<2, 7, DIR> {};
 template <int DIR> struct DFT<15, DIR> : DFTComp<3, 5, DIR> {};
 template <int DIR> struct DFT<16, DIR> : DFTComp<4, 4, DIR> {};
+template <int DIR> struct DFT<32, DIR> : DFTComp<8, 4, DIR> {};
+
+// ---------------------------------------------------------------------------
+// quad (4-lane) exchanges by DPP: lane l reads lane l ^ 1 or l ^ 2 of its quad
+// ---------------------------------------------------------------------------
+template <int X>
+__device__ __forceinline__ float quad_xor(float v) {
+  static_assert(X == 1 || X == 2, "quad partner");
+  // quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E
+  constexpr int CTRL = (X == 1) ? 0xB1 : 0x4E;
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// radix-2 butterfly between lanes l and l ^ X: the lane with bit X clear
+// (hi = false) gets a + p, the other p - a (a: own value, p: partner's)
+template <int X>
+__device__ __forceinline__ float2 quad_bfly(float2 a, float sgn) {
+  const float px = quad_xor<X>(a.x), py = quad_xor<X>(a.y);
+  return make_float2(fmaf(sgn, a.x, px), fmaf(sgn, a.y, py));
+}
 
 // ---------------------------------------------------------------------------
 // FFT plans: the radix sequence for a line length N (product = N)

@@ -717,6 +717,253 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
 }
 
 // ---------------------------------------------------------------------------
+// column pass, register-resident form for long columns, N = 64*M with M = 16
+// or 32 (N = 1024, 2048).  A block of 1024 threads owns 16 adjacent columns,
+// so every row segment it reads or writes is 128 bytes; the tile (256 KiB at
+// N = 2048, more than the LDS holds) stays in registers, M elements per
+// thread, and the column transform is split as N = 64 * M (four-step):
+//   A layout (global loads and stores): thread tid holds column tid % 16,
+//     rows n1 + 64*n2 (n1 = tid / 16, n2 = 0..M-1); an M-point DFT over n2 in
+//     registers, then the twiddle W_N^(n1*k2);
+//   exchange through LDS in rounds of 16 values of k2 (128 KiB per round);
+//   B layout: wavefront wv of round r holds k2 = 16r + wv; the lanes of a
+//     quad (j = lane % 4) hold column lane / 4, slot m <-> n1 = j + 4m; a
+//     16-point DFT over m in registers, the twiddle W_64^(j*c), and a 4-point
+//     DFT across the quad by two DPP butterflies.  Slot c of lane
+//     j = j0 + 2*j1 then holds frequency ky = k2 + M*(c + 16*(j1 + 2*j0)).
+// The phase plate (INIT), propagator, readouts and the exit store act in the
+// B layout on row-major tables (a wavefront touches four 128-byte row
+// segments per access); the inverse transform runs the steps backwards.
+// grid: (ceil(nx/16), nwave); block: 1024
+// ---------------------------------------------------------------------------
+// base[idx] addressed by a 32-bit byte offset (idx * sizeof(T) < 2^32), so
+// that a wavefront-uniform base stays in scalar registers
+template <class T>
+__device__ __forceinline__ T* at32(T* base, int idx) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)(idx * (int)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const T* at32(const T* base, int idx) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)(idx * (int)sizeof(T)));
+}
+
+// v[k] *= W_N^(DIR * n1 * k), k = 1..M-1, N = 64*M, from two exact table
+// reads per factor: W^(n1*(8a + b)) = W^(8a*n1) * W^(b*n1) (n1*k < N)
+template <int M, int DIR>
+__device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int n1) {
+  float2 wb[8];
+  static_for<1, 8>([&](auto B) {
+    constexpr int b = decltype(B)::value;
+    wb[b] = tw[b * n1];
+  });
+  static_for<0, M / 8>([&](auto A) {
+    constexpr int a8 = decltype(A)::value;
+    const float2 wa = tw[8 * a8 * n1];
+    static_for<0, 8>([&](auto B) {
+      constexpr int b = decltype(B)::value;
+      constexpr int k = 8 * a8 + b;
+      if constexpr (k > 0) {
+        const float2 wk = (a8 == 0) ? wb[b] : ((b == 0) ? wa : cmul(wa, wb[b]));
+        v[k] = (DIR < 0) ? cmul(v[k], wk) : cmulconj(v[k], wk);
+      }
+    });
+  });
+}
+
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
+__global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
+  constexpr int M = N / 64;
+  static_assert(M == 16 || M == 32, "long-column kernel: N = 1024 or 2048");
+  constexpr int R = M / 16;   // exchange rounds
+  constexpr int CT = 16;      // columns per block
+  __shared__ float2 s_x[16 * 64 * CT];  // one round: [k2 % 16][n1][column]
+  __shared__ float2 s_tw[N];
+  __shared__ float s_red[16 * kMaxDet];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N; e += 1024) s_tw[e] = a.twg[e];
+  const long long w = blockIdx.y;
+  const int nx = a.nx;
+  const int x0 = blockIdx.x * CT;
+  const int ncol = (nx - x0 < CT) ? (nx - x0) : CT;
+  float2* wbase = a.wave + w * a.wave_stride + x0;
+  // A layout (columns past the grid edge work on column 0 of the tile and
+  // are not stored)
+  const int ca = tid % CT, n1 = tid / CT;
+  const int xa = (ca < ncol) ? ca : 0;
+  // B layout
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int j = lane & 3, cb = lane >> 2;
+  const int j0 = j & 1, j1 = j >> 1;
+  const int dq = j1 + 2 * j0;
+  const float sg1 = j0 ? -1.f : 1.f, sg2 = j1 ? -1.f : 1.f;
+  const bool validb = cb < ncol;
+  const int xb = x0 + (validb ? cb : 0);
+  // slot (r, c) of the B layout <-> row ky = 16r + M*c (uniform part) + kyb
+  const int kyb = wv + 16 * M * dq;
+  auto kyof = [&](int r, int c) { return 16 * r + M * c + kyb; };
+  // rows of the B layout are addressed as (row 16r + M*c of a table) +
+  // (per-lane offset kyb*nx + x): the first part is wavefront-uniform
+  const int ob = kyb * nx + xb;
+  const long long rstep = (long long)M * nx;  // elements from slot c to c + 1
+  // A layout: row n1 + 64*n2 = (uniform row 64*n2) + per-thread offset
+  const long long astep = 64LL * nx;
+  float2 v[M];  // A: v[n2] (rows n1 + 64*n2); B: v[16r + c]
+  if constexpr (!INIT) {
+    static_for<0, M>([&](auto I) {
+      constexpr int n2 = decltype(I)::value;
+      v[n2] = ldg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * nx + xa));
+    });
+    DFT<M, -1>::run(v);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  __syncthreads();  // twiddle table complete
+  if constexpr (!INIT) twiddle_a<M, -1>(v, s_tw, n1);
+  float acc[kMaxDet];
+#pragma unroll
+  for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
+  float dx = 0.f, dy = 0.f, fz = 0.f;
+  if constexpr (INIT) {
+    if (a.ppar) {
+      const float4 q4 = a.ppar[w];
+      dx = q4.x; dy = q4.y; fz = q4.z;
+    }
+  }
+  // B phase, one round (16 values of k2) at a time
+  static_for<0, R>([&](auto RR) {
+    constexpr int r = decltype(RR)::value;
+    float2* u = &v[16 * r];
+    if constexpr (INIT) {
+      const float2* s = a.src + w * a.src_stride + 16 * r * nx;
+      const float kx = (float)fidx(xb, nx) * a.kscx;
+      static_for<0, 16>([&](auto K) {
+        constexpr int c = decltype(K)::value;
+        const int ky = kyof(r, c);
+        float2 val = *at32(s + c * rstep, ob);
+        if (a.ppar) {
+          const float kyf = (float)fidx(ky, N) * a.kscy;
+          const float ksqr = kx * kx + kyf * kyf;
+          float chi = -6.28318531f * (dx * kx + dy * kyf + fz * ksqr);
+          if (a.cppl) chi -= *at32(a.cppl + 16 * r * nx + c * rstep, ob);
+          float sn, cs;
+          sincosf(chi, &sn, &cs);
+          val = cmul(val, make_float2(cs, sn));
+        }
+        u[c] = val;
+      });
+    } else {
+      if constexpr (r > 0) __syncthreads();  // the previous round's reads are done
+      static_for<0, 16>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        s_x[(k * 64 + n1) * CT + ca] = u[k];
+      });
+      __syncthreads();
+      static_for<0, 16>([&](auto K) {
+        constexpr int m = decltype(K)::value;
+        u[m] = s_x[(wv * 64 + j + 4 * m) * CT + cb];
+      });
+      DFT<16, -1>::run(u);
+      __builtin_amdgcn_sched_barrier(0);
+      static_for<1, 16>([&](auto K) {
+        constexpr int c = decltype(K)::value;
+        u[c] = cmul(u[c], s_tw[j * c * M]);  // W_64^(j*c)
+      });
+      static_for<0, 16>([&](auto K) {
+        constexpr int c = decltype(K)::value;
+        float2 z = quad_bfly<2>(u[c], sg2);
+        if (j0 & j1) z = mul_si<-1>(z);  // W_4^(j0*e0)
+        u[c] = cmul(quad_bfly<1>(z, sg1), *at32(a.pro + 16 * r * nx + c * rstep, ob));
+      });
+    }
+    if constexpr (DET) {
+      if (validb) {
+        // de-scan (DescanDifN, JMultiSlice.cpp:2184-2275): pixel i is
+        // weighted with det(i - d)
+        const int xs = wrap_sub(xb, a.dsx, nx);
+        for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
+          const float* dw = a.detw + d * (long long)N * nx;
+          float wt[16];
+          static_for<0, 16>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            wt[c] = *at32(dw, ((kyof(r, c) - a.dsy) & (N - 1)) * nx + xs);
+          });
+          float sacc = 0.f;
+          static_for<0, 16>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            sacc += (u[c].x * u[c].x + u[c].y * u[c].y) * wt[c];
+          });
+#pragma unroll
+          for (int dd = 0; dd < kMaxDet; ++dd)
+            if (dd == d) acc[dd] += sacc;
+        }
+        if (a.dif || a.wff || a.dif_avg || a.wff_avg) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
+          const float wg = a.cwgt[w];
+          const long long cbase = w * a.chan_wstride + a.chan_off + wrap_sub(xb, a.dsx, nx);
+          static_for<0, 16>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            const long long idx = cbase + ((kyof(r, c) - a.dsy) & (N - 1)) * nx;
+            const float2 z = u[c];
+            const float pz = z.x * z.x + z.y * z.y;
+            if (a.dif) a.dif[idx] += wg * pz;
+            if (a.dif_avg) a.dif_avg[idx] += wg * pz;
+            if (a.wff) { float2 o = a.wff[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff[idx] = o; }
+            if (a.wff_avg) { float2 o = a.wff_avg[idx]; o.x += z.x * wg; o.y += z.y * wg; a.wff_avg[idx] = o; }
+          });
+        }
+      }
+    }
+    if constexpr (EXIT) {
+      if (validb) {
+        float2* wv0 = a.wave + w * a.wave_stride + 16 * r * nx;
+        static_for<0, 16>([&](auto K) {
+          constexpr int c = decltype(K)::value;
+          stg<C_::NT_COL>(at32(wv0 + c * rstep, ob), u[c]);
+        });
+      }
+    }
+  });
+  if constexpr (DET)
+    block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + blockIdx.x,
+                     a.part_dstride);
+  if constexpr (!EXIT) {
+    static_for<0, R>([&](auto RR) {
+      constexpr int r = decltype(RR)::value;
+      float2* u = &v[16 * r];
+      static_for<0, 16>([&](auto K) {
+        constexpr int c = decltype(K)::value;
+        float2 z = quad_bfly<1>(u[c], sg1);
+        if (j0 & j1) z = mul_si<+1>(z);
+        u[c] = quad_bfly<2>(z, sg2);
+      });
+      static_for<1, 16>([&](auto K) {
+        constexpr int c = decltype(K)::value;
+        u[c] = cmulconj(u[c], s_tw[j * c * M]);
+      });
+      DFT<16, +1>::run(u);
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();  // earlier users of the exchange buffer are done
+      static_for<0, 16>([&](auto K) {
+        constexpr int m = decltype(K)::value;
+        s_x[(wv * 64 + j + 4 * m) * CT + cb] = u[m];
+      });
+      __syncthreads();
+      static_for<0, 16>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        u[k] = s_x[(k * 64 + n1) * CT + ca];
+      });
+    });
+    __builtin_amdgcn_sched_barrier(0);
+    twiddle_a<M, +1>(v, s_tw, n1);
+    DFT<M, +1>::run(v);
+    if (ca < ncol) {
+      static_for<0, M>([&](auto I) {
+        constexpr int n2 = decltype(I)::value;
+        stg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * nx + ca), v[n2]);
+      });
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // standalone line transforms (2-D FFT = row transform + column transform)
 // ---------------------------------------------------------------------------
 template <int N, int DIR, class C_ = Cfg<N>>
@@ -851,7 +1098,20 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   a.detw_t = h.detw_t;
   a.cppl_t = h.cppl_t;
   using CF = Cfg<N>;
-  if constexpr (CF::WL) {
+  if constexpr (CF::XL) {
+    dim3 g((h.nx + 15) / 16, nwave), b(1024);
+    if (h.detect) {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, true>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, true>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_x<N, false, false, true>), g, b, 0, st, a);
+    } else {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, false>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, false>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, false>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_x<N, false, false, false>), g, b, 0, st, a);
+    }
+  } else if constexpr (CF::WL) {
     dim3 g((h.nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
     // detection planes get their own instantiation: the readout's registers
     // would otherwise halve the occupancy of every other slice
