@@ -62,7 +62,14 @@ struct Cfg {
   // row pass keeps the table reads
   static constexpr int SW = 0;
   static constexpr bool TWR = true;
-  static constexpr bool TWR_ROW = false;
+  // Long rows (N >= 1024): the row pass reads its twiddles through the caches
+  // from the global table (GTW_ROW) instead of an LDS copy that every block
+  // fills (16 KiB per 16 KiB row at N = 2048), with one read per butterfly
+  // and the product tree (TWR_ROW); per 1 GiB of waves on MI355X:
+  // N = 2048 0.535 -> 0.392 ms, N = 1024 0.417 -> 0.365 ms, N = 512
+  // 0.351 -> 0.348 ms (kept as is)
+  static constexpr bool TWR_ROW = N >= 1024;
+  static constexpr bool GTW_ROW = N >= 1024;
   // fused tile I/O in the wave-local column kernel where the plan allows it
   static constexpr bool FIO = true;
 };

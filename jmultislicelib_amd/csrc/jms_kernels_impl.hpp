@@ -98,11 +98,15 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
-  __shared__ float2 s_tw[N];
+  // twiddles from an LDS copy of the table, or (GTW_ROW) read through the
+  // caches from the global table
+  __shared__ float2 s_tw[C_::GTW_ROW ? 1 : N];
   __shared__ float2 s_buf[L::words(LPB)];
   const int tid = threadIdx.x;
   const int t = tid % T, line = tid / T;
-  for (int e = tid; e < N; e += T * LPB) s_tw[e] = twg[e];
+  if constexpr (!C_::GTW_ROW)
+    for (int e = tid; e < N; e += T * LPB) s_tw[e] = twg[e];
+  const float2* tw = C_::GTW_ROW ? twg : s_tw;
   const long long w = blockIdx.y;
   const int row = blockIdx.x * LPB + line;
   const bool valid = row < ny;
@@ -124,10 +128,10 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
       }
     });
   }
-  __syncthreads();
+  if constexpr (!C_::GTW_ROW) __syncthreads();  // twiddle table complete
   // rows lie inside one wavefront when T divides 64: wave-level barriers suffice
   using RowSync = std::conditional_t<(64 % T == 0), SyncWave, SyncBlock>;
-  LineFFT<N, T, +1, P, false, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, +1, P, false, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, tw, t, RowSync{});
   if constexpr (MODE >= 1) {
     if (valid) {  // weight * |psi(r)|^2 and weight * psi(r) into the channels of this plane
       const float wg = rc.cwgt[w];
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
       }
     });
   }
-  LineFFT<N, T, -1, P, true, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, s_tw, t, RowSync{});
+  LineFFT<N, T, -1, P, true, L, NREG, C_::TWR_ROW>::run(v, s_buf, line, tw, t, RowSync{});
   if (valid) {
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
