@@ -740,6 +740,14 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
 // segments per access); the inverse transform runs the steps backwards.
 // grid: (ceil(nx/16), nwave); block: 1024
 // ---------------------------------------------------------------------------
+// the value of x, hidden from the optimiser: table reads indexed with it are
+// repeated where they are needed instead of being kept live in registers
+// from an earlier identical read
+__device__ __forceinline__ int opaque(int x) {
+  __asm__ volatile("" : "+v"(x));
+  return x;
+}
+
 // base[idx] addressed by a 32-bit byte offset (idx * sizeof(T) < 2^32), so
 // that a wavefront-uniform base stays in scalar registers
 template <class T>
@@ -821,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   }
   __builtin_amdgcn_sched_barrier(0);
   __syncthreads();  // twiddle table complete
-  if constexpr (!INIT) twiddle_a<M, -1>(v, s_tw, n1);
+  if constexpr (!INIT) twiddle_a<M, -1>(v, s_tw, opaque(n1));
   float acc[kMaxDet];
 #pragma unroll
   for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
@@ -867,9 +875,10 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       });
       DFT<16, -1>::run(u);
       __builtin_amdgcn_sched_barrier(0);
+      const int jt = opaque(j) * M;
       static_for<1, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
-        u[c] = cmul(u[c], s_tw[j * c * M]);  // W_64^(j*c)
+        u[c] = cmul(u[c], s_tw[jt * c]);  // W_64^(j*c)
       });
       static_for<0, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
@@ -883,12 +892,14 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
         // de-scan (DescanDifN, JMultiSlice.cpp:2184-2275): pixel i is
         // weighted with det(i - d)
         const int xs = wrap_sub(xb, a.dsx, nx);
+        const int qs = (kyb - a.dsy) & (N - 1);  // row of slot (r, c): (16r + M*c + qs) mod N
         for (int d = 0; d < a.ndet && d < kMaxDet; ++d) {  // uniform loop over detectors
           const float* dw = a.detw + d * (long long)N * nx;
+          const int q = opaque(qs);  // row offsets recomputed per detector
           float wt[16];
           static_for<0, 16>([&](auto K) {
             constexpr int c = decltype(K)::value;
-            wt[c] = *at32(dw, ((kyof(r, c) - a.dsy) & (N - 1)) * nx + xs);
+            wt[c] = *at32(dw, ((16 * r + M * c + q) & (N - 1)) * nx + xs);
           });
           float sacc = 0.f;
           static_for<0, 16>([&](auto K) {
@@ -901,10 +912,11 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
         }
         if (a.dif || a.wff || a.dif_avg || a.wff_avg) {  // ReadoutDifDet_d channels (JMultiSlice.cpp:4310-4356)
           const float wg = a.cwgt[w];
-          const long long cbase = w * a.chan_wstride + a.chan_off + wrap_sub(xb, a.dsx, nx);
+          const long long cbase = w * a.chan_wstride + a.chan_off;
+          const int q = opaque(qs);
           static_for<0, 16>([&](auto K) {
             constexpr int c = decltype(K)::value;
-            const long long idx = cbase + ((kyof(r, c) - a.dsy) & (N - 1)) * nx;
+            const long long idx = cbase + ((16 * r + M * c + q) & (N - 1)) * nx + xs;
             const float2 z = u[c];
             const float pz = z.x * z.x + z.y * z.y;
             if (a.dif) a.dif[idx] += wg * pz;
@@ -938,9 +950,10 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
         if (j0 & j1) z = mul_si<+1>(z);
         u[c] = quad_bfly<2>(z, sg2);
       });
+      const int jt = opaque(j) * M;
       static_for<1, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
-        u[c] = cmulconj(u[c], s_tw[j * c * M]);
+        u[c] = cmulconj(u[c], s_tw[jt * c]);
       });
       DFT<16, +1>::run(u);
       __builtin_amdgcn_sched_barrier(0);
@@ -956,7 +969,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       });
     });
     __builtin_amdgcn_sched_barrier(0);
-    twiddle_a<M, +1>(v, s_tw, n1);
+    twiddle_a<M, +1>(v, s_tw, opaque(n1));
     DFT<M, +1>::run(v);
     if (ca < ncol) {
       static_for<0, M>([&](auto I) {
