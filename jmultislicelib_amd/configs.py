@@ -237,3 +237,76 @@ def setup_from_slice_files(paths, nobj: int, ndetper: int = 0, alpha: float = 25
                  nvarslc=[int(h.num_var) for h in heads], gratings=gratings, propagators=pros,
                  proidx=list(range(nslc)), objslc=objslc, det_objslc=det_objslc, ndetslc=ndetslc,
                  detectors=dets, probe=probe, flags=flags, slice_thickness=float(h0.sz))
+
+
+@dataclass
+class Job:
+    """One job of an uneven batch: its set-up and probe positions [nm]."""
+    setup: Setup
+    dx: np.ndarray
+    dy: np.ndarray
+    cell_ang: tuple  # lattice cell used (a, b, c) [A]
+    species: np.ndarray
+
+
+JOB_EDGES = (360, 384, 480, 512, 640, 720)
+
+
+def random_job(rng: np.random.Generator, edge: int = None, nslices: int = None, npos: int = None,
+               sampling_ang: float = 0.06, ht: float = 200.0, det_ranges=DETECTOR_RANGES,
+               flags: int = 1) -> Job:
+    """A random orthorhombic crystal job: grid edge from JOB_EDGES at
+    sampling_ang, a, b, c uniform in [3.5, 6.0] A, 1-4 species with Z uniform
+    in [6, 79], 2-8 atoms per cell at uniform fractional positions, the
+    Gaussian potential model, two slice types of c/2 (fractional z below /
+    above 1/2) alternating for nslices in [50, 400], 16-256 probe positions
+    on a raster over one cell, a 20-30 mrad aperture.  The square super-cell
+    (edge * sampling_ang) holds a whole number of cells along x and y; the
+    cell is stretched to fit it."""
+    e = int(edge if edge is not None else rng.choice(JOB_EDGES))
+    a, b, c = rng.uniform(3.5, 6.0, 3)
+    nsp = int(rng.integers(1, 5))
+    species = rng.integers(6, 80, nsp).astype(np.int32)
+    nat = int(rng.integers(2, 9))
+    frac = rng.uniform(0.0, 1.0, (nat, 3))
+    kind = rng.integers(0, nsp, nat)
+    ns = int(nslices if nslices is not None else rng.integers(50, 401))
+    np_ = int(npos if npos is not None else rng.integers(16, 257))
+    alpha = float(rng.uniform(20.0, 30.0))
+    L = e * sampling_ang
+    ncx, ncy = max(1, int(round(L / a))), max(1, int(round(L / b)))
+    ax, by = L / ncx, L / ncy
+    dz_nm = 0.5 * c * 0.1
+    J = JMultiSlice()
+    J.SetHighTension(ht)
+    J.SetGridSize(e, e)
+    J.SetSupercellSize(L * 0.1, L * 0.1, c * 0.1)
+    wl = J.GetWaveLength()
+    gratings = []
+    for t in (0, 1):
+        sel = (frac[:, 2] < 0.5) if t == 0 else (frac[:, 2] >= 0.5)
+        fx, fy, zs = frac[sel, 0], frac[sel, 1], species[kind[sel]]
+        iy, ix = np.meshgrid(np.arange(ncy), np.arange(ncx), indexing="ij")
+        ox, oy = ix.ravel() * ax, iy.ravel() * by
+        xy = np.stack([(ox[:, None] + fx[None, :] * ax).ravel(), (oy[:, None] + fy[None, :] * by).ravel()], 1)
+        z = np.tile(zs, len(ox)).astype(np.int32)
+        gratings.append(gaussian_grating(e, L, np.ascontiguousarray(xy, np.float64), z)[None])
+    pro = J.CalculatePropagator(dz_nm, 0.0, 0.0, 1)
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in det_ranges]
+    probe = J.CalculateProbeWaveFourier(alpha, 0.03)
+    objslc = np.array([i % 2 for i in range(ns)], np.int32)
+    det_objslc, ndetslc = detection_planes(0, ns)
+    sp = Setup(nx=e, ny=e, a0=(L * 0.1, L * 0.1, c * 0.1), ht=ht, wl=wl, nvarslc=[1, 1], gratings=gratings,
+               propagators=[pro], proidx=[0, 0], objslc=objslc, det_objslc=det_objslc, ndetslc=ndetslc,
+               detectors=dets, probe=probe, flags=flags, slice_thickness=dz_nm)
+    side = int(np.ceil(np.sqrt(np_)))
+    i = np.arange(np_)
+    dx = ((i % side) * (ax * 0.1 / side)).astype(np.float32)
+    dy = ((i // side) * (by * 0.1 / side)).astype(np.float32)
+    return Job(setup=sp, dx=dx, dy=dy, cell_ang=(ax, by, c), species=species)
+
+
+def uneven_jobs(njobs: int = 64, seed: int = 2, **kw) -> list:
+    """The uneven batch: njobs random crystal jobs from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    return [random_job(rng, **kw) for _ in range(njobs)]
