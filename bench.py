@@ -156,6 +156,7 @@ def run_jobs64(args):
     for j in jobs:
         eng = Engine().setup(j.setup)
         eng.set_batch(len(j.dx))
+        eng.set_graph(args.graph)
         var = np.zeros((len(j.dx), len(j.setup.objslc)), np.int32)
         eng.stage_scan(j.dx, j.dy, None, np.ones(1, np.float32), var)
         engines.append(eng)
