@@ -22,6 +22,7 @@ c_double = C.c_double
 c_void_p = C.c_void_p
 c_char_p = C.c_char_p
 c_longlong = C.c_longlong
+c_uint = C.c_uint
 
 # (name, restype, argtypes) of every exported entry point
 _SIGS = [
@@ -136,6 +137,21 @@ _SIGS = [
                                      c_int, c_void_p, c_void_p]),
     ("jms_gaussian_phase_grating", c_int, [c_int, c_int, c_double, c_double, c_int, c_void_p, c_void_p,
                                            c_double, c_double, c_void_p]),
+    # surrogate network (include/jms_nn.h)
+    ("jms_nn_create", c_int, [c_void_p, c_int, c_uint]),
+    ("jms_nn_destroy", None, [c_void_p]),
+    ("jms_nn_num_params", c_int, []),
+    ("jms_nn_get_params", c_int, [c_void_p, c_void_p]),
+    ("jms_nn_set_params", c_int, [c_void_p, c_void_p]),
+    ("jms_nn_forward", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    ("jms_nn_gradients", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("jms_nn_data_create", c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    ("jms_nn_data_destroy", None, [c_void_p]),
+    ("jms_nn_data_windows", c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_float, c_float, c_void_p]),
+    ("jms_nn_data_get", c_int, [c_void_p, c_void_p]),
+    ("jms_nn_train_epoch", c_int, [c_void_p, c_void_p, c_int, c_float, c_void_p, c_uint, c_void_p]),
+    ("jms_nn_predict", c_int, [c_void_p, c_void_p, c_void_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
