@@ -115,8 +115,9 @@ __global__ void k_conv_bwd_data(const float* __restrict__ dy, const uint16_t* __
   dx[i] = acc;
 }
 
-// block (co, ci): dw[co][ci][ky][kx] = sum_{n,oy,ox} dy[n][co][oy][ox] x[n][ci][iy][ix];
-// the ci = 0 block also writes db[co] = sum dy[n][co][.][.]
+// block (co, ci, part): the part-th share of the samples adds to
+// dw[co][ci][ky][kx] = sum_{n,oy,ox} dy[n][co][oy][ox] x[n][ci][iy][ix]
+// (atomically; dw is cleared first); the ci = 0 blocks add db[co] = sum dy
 template <int S>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ dy, const uint16_t* __restrict__ x,
                                                     float* __restrict__ dw, float* __restrict__ db, int n,
@@ -126,8 +127,9 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ dy
 #pragma unroll
   for (int k = 0; k < 10; ++k) acc[k] = 0.f;
   const int per = hout * hout;
-  const long long tot = (long long)n * per;
-  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+  const int s0 = (int)((long long)n * blockIdx.y / gridDim.y), s1 = (int)((long long)n * (blockIdx.y + 1) / gridDim.y);
+  const long long tot = (long long)s1 * per;
+  for (long long e = (long long)s0 * per + threadIdx.x; e < tot; e += blockDim.x) {
     const long long s = e / per;
     const int r = (int)(e % per), oy = r / hout, ox = r % hout;
     const float g = dy[(s * CH + co) * per + r];
@@ -158,8 +160,8 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ dy
   if (threadIdx.x < 10) {
     const int k = threadIdx.x;
     const float v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
-    if (k < 9) dw[((long long)co * ci_n + ci) * 9 + k] = v;
-    else if (ci == 0) db[co] = v;
+    if (k < 9) atomicAdd(&dw[((long long)co * ci_n + ci) * 9 + k], v);
+    else if (ci == 0) atomicAdd(&db[co], v);
   }
 }
 
@@ -241,6 +243,11 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   const float q = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
   p[i] = q;
   pbf[i] = f2bf(q);
+}
+
+__global__ void k_zero(float* __restrict__ p, int np) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) p[i] = 0.f;
 }
 
 __global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ q, int np) {
@@ -372,14 +379,17 @@ hipError_t backward(jms_nn* m, int n) {
                      m->act[NCONV], n, FLAT, HID, m->g + OFF_D1W, m->g + OFF_D1B);
   hipLaunchKernelGGL(k_dense_bwd_data, dim3(nblk((long long)n * FLAT, 256)), dim3(256), 0, m->st, gh,
                      m->pbf + OFF_D1W, m->act[NCONV], n, FLAT, HID, m->gbuf[NCONV - 1]);
-  // convolutions, last to first; gbuf[l] = gradient w.r.t. the output of conv l+1
+  // convolutions, last to first; gbuf[l] = gradient w.r.t. the output of conv l+1;
+  // their weight gradients are accumulated atomically over shares of the batch
+  hipLaunchKernelGGL(k_zero, dim3(nblk(OFF_D1W, 256)), dim3(256), 0, m->st, m->g, OFF_D1W);
   for (int l = NCONV - 1; l >= 0; --l) {
     const float* dy = m->gbuf[l];
+    const int parts = std::max(1, std::min(n, 4096 / (CH * CIN[l])));
     if (STRIDE[l] == 1)
-      hipLaunchKernelGGL(k_conv_bwd_w<1>, dim3(CH * CIN[l]), dim3(256), 0, m->st, dy, m->act[l],
+      hipLaunchKernelGGL(k_conv_bwd_w<1>, dim3(CH * CIN[l], parts), dim3(256), 0, m->st, dy, m->act[l],
                          m->g + off_cw(l), m->g + off_cb(l), n, CIN[l], HIN[l], HOUT[l]);
     else
-      hipLaunchKernelGGL(k_conv_bwd_w<2>, dim3(CH * CIN[l]), dim3(256), 0, m->st, dy, m->act[l],
+      hipLaunchKernelGGL(k_conv_bwd_w<2>, dim3(CH * CIN[l], parts), dim3(256), 0, m->st, dy, m->act[l],
                          m->g + off_cw(l), m->g + off_cb(l), n, CIN[l], HIN[l], HOUT[l]);
     if (l > 0) {
       const long long tot = (long long)n * CIN[l] * HIN[l] * HIN[l];
