@@ -44,124 +44,144 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// y[n][co][oy][ox] = act(b[co] + sum_{ci,ky,kx} x[n][ci][oy*S+ky-1][ox*S+kx-1] w[co][ci][ky][kx])
-template <int S>
-__global__ void k_conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
-                           const float* __restrict__ b, uint16_t* __restrict__ y, int n, int ci_n, int hin,
-                           int hout) {
+constexpr int GMAX = 8;  // channels per thread at most (a wavefront-uniform channel group)
+
+// y[n][co][oy][ox] = relu(b[co] + sum_{ci,ky,kx} x[n][ci][oy*S+ky-1][ox*S+kx-1] w[co][ci][ky][kx]);
+// thread (n, oy, ox) computes the G output channels of group blockIdx.y from
+// one input read per tap
+template <int S, int G>
+__global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+                                                  const float* __restrict__ b, uint16_t* __restrict__ y, int n,
+                                                  int ci_n, int hin, int hout) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long tot = (long long)n * CH * hout * hout;
+  const int co0 = blockIdx.y * G;
+  float acc[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) acc[j] = b[co0 + j];
+  const long long tot = (long long)n * hout * hout;
   if (i >= tot) return;
   const int ox = (int)(i % hout), oy = (int)((i / hout) % hout);
-  const int co = (int)((i / ((long long)hout * hout)) % CH);
-  const long long s = i / ((long long)hout * hout * CH);
-  float acc = b[co];
+  const long long s = i / ((long long)hout * hout);
   const uint16_t* xs = x + s * ci_n * hin * hin;
-  const uint16_t* wc = w + (long long)co * ci_n * 9;
   for (int ci = 0; ci < ci_n; ++ci) {
     const uint16_t* xc = xs + (long long)ci * hin * hin;
 #pragma unroll
     for (int ky = 0; ky < 3; ++ky) {
       const int iy = oy * S + ky - 1;
-      if (iy < 0 || iy >= hin) continue;
 #pragma unroll
       for (int kx = 0; kx < 3; ++kx) {
         const int ix = ox * S + kx - 1;
-        if (ix < 0 || ix >= hin) continue;
-        acc = fmaf(bf2f(xc[iy * hin + ix]), bf2f(wc[ci * 9 + ky * 3 + kx]), acc);
+        const float xv = (iy >= 0 && iy < hin && ix >= 0 && ix < hin) ? bf2f(xc[iy * hin + ix]) : 0.f;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+          acc[j] = fmaf(xv, bf2f(w[((co0 + j) * ci_n + ci) * 9 + ky * 3 + kx]), acc[j]);
       }
     }
   }
-  y[i] = f2bf(fmaxf(acc, 0.f));
+  const long long hw = (long long)hout * hout;
+#pragma unroll
+  for (int j = 0; j < G; ++j) y[(s * CH + co0 + j) * hw + oy * hout + ox] = f2bf(fmaxf(acc[j], 0.f));
 }
 
 // dx[n][ci][iy][ix] = [a[n][ci][iy][ix] > 0] * sum dy[n][co][oy][ox] w[co][ci][ky][kx]
-// over iy = oy*S + ky - 1, ix = ox*S + kx - 1
-template <int S>
-__global__ void k_conv_bwd_data(const float* __restrict__ dy, const uint16_t* __restrict__ w,
-                                const uint16_t* __restrict__ a, float* __restrict__ dx, int n, int ci_n,
-                                int hin, int hout) {
+// over iy = oy*S + ky - 1, ix = ox*S + kx - 1; thread (n, iy, ix) computes the
+// G input channels of group blockIdx.y from one dy read per tap
+template <int S, int G>
+__global__ __launch_bounds__(256) void k_conv_bwd_data(const float* __restrict__ dy, const uint16_t* __restrict__ w,
+                                                       const uint16_t* __restrict__ a, float* __restrict__ dx, int n,
+                                                       int ci_n, int hin, int hout) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long tot = (long long)n * ci_n * hin * hin;
+  const int ci0 = blockIdx.y * G;
+  const long long tot = (long long)n * hin * hin;
   if (i >= tot) return;
-  if (bf2f(a[i]) <= 0.f) {
-    dx[i] = 0.f;
-    return;
-  }
   const int ix = (int)(i % hin), iy = (int)((i / hin) % hin);
-  const int ci = (int)((i / ((long long)hin * hin)) % ci_n);
-  const long long s = i / ((long long)hin * hin * ci_n);
+  const long long s = i / ((long long)hin * hin);
   const float* ds = dy + s * CH * hout * hout;
-  float acc = 0.f;
-  for (int co = 0; co < CH; ++co) {
-    const uint16_t* wc = w + ((long long)co * ci_n + ci) * 9;
-    const float* dc = ds + (long long)co * hout * hout;
+  float acc[G];
 #pragma unroll
-    for (int ky = 0; ky < 3; ++ky) {
-      const int ty = iy + 1 - ky;
-      if (ty < 0 || ty % S) continue;
-      const int oy = ty / S;
-      if (oy >= hout) continue;
+  for (int j = 0; j < G; ++j) acc[j] = 0.f;
 #pragma unroll
-      for (int kx = 0; kx < 3; ++kx) {
-        const int tx = ix + 1 - kx;
-        if (tx < 0 || tx % S) continue;
-        const int ox = tx / S;
-        if (ox >= hout) continue;
-        acc = fmaf(dc[oy * hout + ox], bf2f(wc[ky * 3 + kx]), acc);
+  for (int ky = 0; ky < 3; ++ky) {
+    const int ty = iy + 1 - ky;
+    if (ty < 0 || ty % S) continue;
+    const int oy = ty / S;
+    if (oy >= hout) continue;
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx) {
+      const int tx = ix + 1 - kx;
+      if (tx < 0 || tx % S) continue;
+      const int ox = tx / S;
+      if (ox >= hout) continue;
+      for (int co = 0; co < CH; ++co) {
+        const float g = ds[((long long)co * hout + oy) * hout + ox];
+#pragma unroll
+        for (int j = 0; j < G; ++j) acc[j] = fmaf(g, bf2f(w[(co * ci_n + ci0 + j) * 9 + ky * 3 + kx]), acc[j]);
       }
     }
   }
-  dx[i] = acc;
+  const long long hw = (long long)hin * hin;
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const long long o = (s * ci_n + ci0 + j) * hw + iy * hin + ix;
+    dx[o] = (bf2f(a[o]) > 0.f) ? acc[j] : 0.f;
+  }
 }
 
-// block (co, ci, part): the part-th share of the samples adds to
-// dw[co][ci][ky][kx] = sum_{n,oy,ox} dy[n][co][oy][ox] x[n][ci][iy][ix]
-// (atomically; dw is cleared first); the ci = 0 blocks add db[co] = sum dy
-template <int S>
+// block (ci, co group, part): the part-th share of the samples adds to
+// dw[co][ci][ky][kx] = sum_{n,oy,ox} dy[n][co][oy][ox] x[n][ci][iy][ix] for
+// the G channels co of the group (atomically; dw is cleared first); the
+// ci = 0 blocks add db[co] = sum dy
+template <int S, int G = GMAX>
 __global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ dy, const uint16_t* __restrict__ x,
                                                     float* __restrict__ dw, float* __restrict__ db, int n,
                                                     int ci_n, int hin, int hout) {
-  const int co = blockIdx.x / ci_n, ci = blockIdx.x % ci_n;
-  float acc[10];
+  const int ci = blockIdx.x % ci_n, co0 = (blockIdx.x / ci_n) * G;
+  float acc[G][10];
 #pragma unroll
-  for (int k = 0; k < 10; ++k) acc[k] = 0.f;
+  for (int j = 0; j < G; ++j)
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[j][k] = 0.f;
   const int per = hout * hout;
   const int s0 = (int)((long long)n * blockIdx.y / gridDim.y), s1 = (int)((long long)n * (blockIdx.y + 1) / gridDim.y);
   const long long tot = (long long)s1 * per;
   for (long long e = (long long)s0 * per + threadIdx.x; e < tot; e += blockDim.x) {
     const long long s = e / per;
     const int r = (int)(e % per), oy = r / hout, ox = r % hout;
-    const float g = dy[(s * CH + co) * per + r];
+    float g[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) g[j] = dy[(s * CH + co0 + j) * per + r];
     const uint16_t* xc = x + (s * ci_n + ci) * hin * hin;
-    acc[9] += g;
+#pragma unroll
+    for (int j = 0; j < G; ++j) acc[j][9] += g[j];
 #pragma unroll
     for (int ky = 0; ky < 3; ++ky) {
       const int iy = oy * S + ky - 1;
-      if (iy < 0 || iy >= hin) continue;
 #pragma unroll
       for (int kx = 0; kx < 3; ++kx) {
         const int ix = ox * S + kx - 1;
-        if (ix < 0 || ix >= hin) continue;
-        acc[ky * 3 + kx] = fmaf(g, bf2f(xc[iy * hin + ix]), acc[ky * 3 + kx]);
+        const float xv = (iy >= 0 && iy < hin && ix >= 0 && ix < hin) ? bf2f(xc[iy * hin + ix]) : 0.f;
+#pragma unroll
+        for (int j = 0; j < G; ++j) acc[j][ky * 3 + kx] = fmaf(g[j], xv, acc[j][ky * 3 + kx]);
       }
     }
   }
-  __shared__ float red[10][4];
+  __shared__ float red[G * 10][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 10; ++k) {
-    float v = acc[k];
+  for (int j = 0; j < G; ++j)
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[k][wv] = v;
-  }
+    for (int k = 0; k < 10; ++k) {
+      float v = acc[j][k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[j * 10 + k][wv] = v;
+    }
   __syncthreads();
-  if (threadIdx.x < 10) {
-    const int k = threadIdx.x;
-    const float v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
-    if (k < 9) atomicAdd(&dw[((long long)co * ci_n + ci) * 9 + k], v);
-    else if (ci == 0) atomicAdd(&db[co], v);
+  if (threadIdx.x < G * 10) {
+    const int j = threadIdx.x / 10, k = threadIdx.x % 10;
+    const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+    if (k < 9) atomicAdd(&dw[((long long)(co0 + j) * ci_n + ci) * 9 + k], v);
+    else if (ci == 0) atomicAdd(&db[co0 + j], v);
   }
 }
 
@@ -298,6 +318,14 @@ __global__ void k_windows(const float* __restrict__ maps, const float* __restric
 
 unsigned int nblk(long long n, int bs) { return (unsigned int)((n + bs - 1) / bs); }
 
+// channels per thread of a conv launch over px pixels and nch channels: as
+// many as keep at least 64k threads in flight (8, 4, 2 or 1)
+int group_size(long long px, int nch) {
+  int g = GMAX;
+  while (g > 1 && px * (nch / g) < 65536) g /= 2;
+  return g;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -337,24 +365,32 @@ hipError_t dalloc(T** p, size_t n) { return hipMalloc((void**)p, n * sizeof(T));
 
 void release(jms_nn* m) {
   if (!m) return;
-  hipFree(m->p); hipFree(m->g); hipFree(m->m); hipFree(m->v); hipFree(m->pbf);
-  for (auto* a : m->act) hipFree(a);
-  for (auto* a : m->gbuf) hipFree(a);
-  hipFree(m->out); hipFree(m->tb); hipFree(m->scale); hipFree(m->loss_part); hipFree(m->idx);
-  if (m->st) hipStreamDestroy(m->st);
+  (void)hipFree(m->p); (void)hipFree(m->g); (void)hipFree(m->m); (void)hipFree(m->v); (void)hipFree(m->pbf);
+  for (auto* a : m->act) (void)hipFree(a);
+  for (auto* a : m->gbuf) (void)hipFree(a);
+  (void)hipFree(m->out); (void)hipFree(m->tb); (void)hipFree(m->scale); (void)hipFree(m->loss_part);
+  (void)hipFree(m->idx);
+  if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
 }
 
 // forward over act[0] (n samples) -> out
 hipError_t forward(jms_nn* m, int n) {
   for (int l = 0; l < NCONV; ++l) {
-    const long long tot = (long long)n * CH * HOUT[l] * HOUT[l];
-    if (STRIDE[l] == 1)
-      hipLaunchKernelGGL(k_conv_fwd<1>, dim3(nblk(tot, 256)), dim3(256), 0, m->st, m->act[l], m->pbf + off_cw(l),
-                         m->p + off_cb(l), m->act[l + 1], n, CIN[l], HIN[l], HOUT[l]);
-    else
-      hipLaunchKernelGGL(k_conv_fwd<2>, dim3(nblk(tot, 256)), dim3(256), 0, m->st, m->act[l], m->pbf + off_cw(l),
-                         m->p + off_cb(l), m->act[l + 1], n, CIN[l], HIN[l], HOUT[l]);
+    const long long px = (long long)n * HOUT[l] * HOUT[l];
+    const int gsz = group_size(px, CH);
+    const dim3 g(nblk(px, 256), CH / gsz);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, g, dim3(256), 0, m->st, m->act[l], m->pbf + off_cw(l), m->p + off_cb(l),
+                         m->act[l + 1], n, CIN[l], HIN[l], HOUT[l]);
+    };
+    if (STRIDE[l] == 1) {
+      if (gsz == 8) go(k_conv_fwd<1, 8>); else if (gsz == 4) go(k_conv_fwd<1, 4>);
+      else if (gsz == 2) go(k_conv_fwd<1, 2>); else go(k_conv_fwd<1, 1>);
+    } else {
+      if (gsz == 8) go(k_conv_fwd<2, 8>); else if (gsz == 4) go(k_conv_fwd<2, 4>);
+      else if (gsz == 2) go(k_conv_fwd<2, 2>); else go(k_conv_fwd<2, 1>);
+    }
   }
   hipLaunchKernelGGL(k_dense_fwd, dim3(nblk((long long)n * HID, 256)), dim3(256), 0, m->st, m->act[NCONV],
                      m->pbf + OFF_D1W, m->p + OFF_D1B, n, FLAT, HID, m->act[NCONV + 1], (float*)nullptr);
@@ -384,21 +420,29 @@ hipError_t backward(jms_nn* m, int n) {
   hipLaunchKernelGGL(k_zero, dim3(nblk(OFF_D1W, 256)), dim3(256), 0, m->st, m->g, OFF_D1W);
   for (int l = NCONV - 1; l >= 0; --l) {
     const float* dy = m->gbuf[l];
-    const int parts = std::max(1, std::min(n, 4096 / (CH * CIN[l])));
+    const int nblk_w = CIN[l] * (CH / GMAX);
+    const int parts = std::max(1, std::min(n, 2048 / nblk_w));
     if (STRIDE[l] == 1)
-      hipLaunchKernelGGL(k_conv_bwd_w<1>, dim3(CH * CIN[l], parts), dim3(256), 0, m->st, dy, m->act[l],
+      hipLaunchKernelGGL(k_conv_bwd_w<1>, dim3(nblk_w, parts), dim3(256), 0, m->st, dy, m->act[l],
                          m->g + off_cw(l), m->g + off_cb(l), n, CIN[l], HIN[l], HOUT[l]);
     else
-      hipLaunchKernelGGL(k_conv_bwd_w<2>, dim3(CH * CIN[l], parts), dim3(256), 0, m->st, dy, m->act[l],
+      hipLaunchKernelGGL(k_conv_bwd_w<2>, dim3(nblk_w, parts), dim3(256), 0, m->st, dy, m->act[l],
                          m->g + off_cw(l), m->g + off_cb(l), n, CIN[l], HIN[l], HOUT[l]);
     if (l > 0) {
-      const long long tot = (long long)n * CIN[l] * HIN[l] * HIN[l];
-      if (STRIDE[l] == 1)
-        hipLaunchKernelGGL(k_conv_bwd_data<1>, dim3(nblk(tot, 256)), dim3(256), 0, m->st, dy, m->pbf + off_cw(l),
-                           m->act[l], m->gbuf[l - 1], n, CIN[l], HIN[l], HOUT[l]);
-      else
-        hipLaunchKernelGGL(k_conv_bwd_data<2>, dim3(nblk(tot, 256)), dim3(256), 0, m->st, dy, m->pbf + off_cw(l),
-                           m->act[l], m->gbuf[l - 1], n, CIN[l], HIN[l], HOUT[l]);
+      const long long px = (long long)n * HIN[l] * HIN[l];
+      const int gsz = group_size(px, CIN[l]);
+      const dim3 g(nblk(px, 256), CIN[l] / gsz);
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, dim3(256), 0, m->st, dy, m->pbf + off_cw(l), m->act[l], m->gbuf[l - 1], n,
+                           CIN[l], HIN[l], HOUT[l]);
+      };
+      if (STRIDE[l] == 1) {
+        if (gsz == 8) go(k_conv_bwd_data<1, 8>); else if (gsz == 4) go(k_conv_bwd_data<1, 4>);
+        else if (gsz == 2) go(k_conv_bwd_data<1, 2>); else go(k_conv_bwd_data<1, 1>);
+      } else {
+        if (gsz == 8) go(k_conv_bwd_data<2, 8>); else if (gsz == 4) go(k_conv_bwd_data<2, 4>);
+        else if (gsz == 2) go(k_conv_bwd_data<2, 2>); else go(k_conv_bwd_data<2, 1>);
+      }
     }
   }
   return hipGetLastError();
@@ -560,7 +604,8 @@ int jms_nn_data_windows(jms_nn_data** out, int ncrys, int nt, int m, const float
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  hipFree(dmaps); hipFree(dcells); hipFree(dcid); hipFree(dpx); hipFree(dpy); hipFree(dbw);
+  (void)hipFree(dmaps); (void)hipFree(dcells); (void)hipFree(dcid); (void)hipFree(dpx); (void)hipFree(dpy);
+  (void)hipFree(dbw);
   if (e != hipSuccess) {
     jms_nn_data_destroy(d);
     return 2;
@@ -577,8 +622,8 @@ int jms_nn_data_get(jms_nn_data* d, uint16_t* x) {
 
 void jms_nn_data_destroy(jms_nn_data* d) {
   if (!d) return;
-  hipFree(d->x);
-  hipFree(d->t);
+  (void)hipFree(d->x);
+  (void)hipFree(d->t);
   delete d;
 }
 
