@@ -32,7 +32,7 @@ typedef struct jms_ems_header {
 } jms_ems_header;
 
 /* read the header (prm_slice_file_info::load_ems_header, .cpp:79-149);
- * 0 = ok, >0 = error */
+ * 0 = ok, >0 = error (4: grid edges outside [1, 16384] or no variant) */
 int jms_ems_read_header(const char* path, jms_ems_header* h);
 /* read the 2*grid_x*grid_y*num_var floats of the data block
  * (prm_slice::load_ems_data, prm_slice.cpp:253-300) */

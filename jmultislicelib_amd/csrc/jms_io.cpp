@@ -79,7 +79,11 @@ extern "C" int jms_ems_read_header(const char* path, jms_ems_header* h) {
   ok = ok && read_at(f, kPosThickness, &h->sz, swap) && read_at(f, kPosHighTension, &h->ekv, swap) &&
        read_at(f, kPosPhysSizeX, &h->sx, swap) && read_at(f, kPosPhysSizeY, &h->sy, swap);
   std::fclose(f);
-  return ok ? 0 : 3;
+  if (!ok) return 3;
+  // grid edges lie in [1, 16384] (the byte-order test relies on it); at
+  // least one variant
+  if (h->grid_x < 1 || h->grid_x > 16384 || h->grid_y < 1 || h->grid_y > 16384 || h->num_var < 1) return 4;
+  return 0;
 }
 
 extern "C" int jms_ems_read_data(const char* path, const jms_ems_header* h, float* data) {

@@ -573,7 +573,9 @@ int jms_nn_data_windows(jms_nn_data** out, int ncrys, int nt, int m, const float
   for (int s = 0; s < n; ++s)
     if (cid[s] < 0 || cid[s] >= ncrys) return 1;
   for (int k = 0; k < 2 * ncrys; ++k)
-    if (!(cells[k] > 0.f)) return 1;
+    if (!(cells[k] > 0.f) || !std::isfinite(cells[k])) return 1;
+  for (int s = 0; s < n; ++s)
+    if (!std::isfinite(px[s]) || !std::isfinite(py[s])) return 1;
   *out = nullptr;
   jms_nn_data* d = new jms_nn_data;
   d->n = n;

@@ -149,3 +149,20 @@ def test_setup_from_slice_files_rejects_mixed_grids(tmp_path):
         configs.setup_from_slice_files([str(tmp_path / "a_001.sli"), str(tmp_path / "a_002.sli")], 4)
     with pytest.raises(RuntimeError):
         jio.read_ems(str(tmp_path / "missing.sli"))
+
+
+def test_truncated_and_invalid_files(tmp_path):
+    d = _data(2, 8, 8)
+    p = str(tmp_path / "t.sli")
+    _build(p, d, 2020052801, alt=0x2000, itab=0x1100)
+    raw = open(p, "rb").read()
+    open(p, "wb").write(raw[:-8])  # one complex value short
+    with pytest.raises(RuntimeError):
+        jio.read_ems(p)
+    q = str(tmp_path / "z.sli")
+    _build(q, _data(1, 4, 4), 2020052801, alt=0x2000, itab=0x1100)
+    raw = bytearray(open(q, "rb").read())
+    raw[4:8] = np.array([0], "<i4").tobytes()  # grid_y = 0
+    open(q, "wb").write(bytes(raw))
+    with pytest.raises(RuntimeError):
+        jio.read_ems(q)
