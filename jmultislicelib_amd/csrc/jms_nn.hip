@@ -234,7 +234,7 @@ __global__ void k_dense_bwd_w(const float* __restrict__ dy, const uint16_t* __re
 // dy = d/dy mean(((y - t)/scale)^2); per-block loss sums into loss_part[blockIdx.x]
 __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t,
                                               const float* __restrict__ scale, int n, float* __restrict__ dy,
-                                              float* __restrict__ loss_part) {
+                                              float* __restrict__ loss_part, float* __restrict__ loss_acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (i < n * NOUT) {
@@ -248,7 +248,11 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
   __syncthreads();
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    const float v = red[0] + red[1] + red[2] + red[3];
+    loss_part[blockIdx.x] = v;
+    if (loss_acc) atomicAdd(loss_acc, v / (float)(n * NOUT));  // mean over the batch
+  }
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -343,6 +347,10 @@ struct jms_nn {
   float* scale = nullptr;
   float* loss_part = nullptr;
   int* idx = nullptr;
+  int* perm = nullptr;       // epoch permutation (device), capacity perm_cap
+  int perm_cap = 0;
+  float* eloss = nullptr;    // per-batch losses of an epoch (device), capacity eloss_cap
+  int eloss_cap = 0;
   long long step = 0;
 };
 
@@ -369,7 +377,7 @@ void release(jms_nn* m) {
   for (auto* a : m->act) (void)hipFree(a);
   for (auto* a : m->gbuf) (void)hipFree(a);
   (void)hipFree(m->out); (void)hipFree(m->tb); (void)hipFree(m->scale); (void)hipFree(m->loss_part);
-  (void)hipFree(m->idx);
+  (void)hipFree(m->idx); (void)hipFree(m->perm); (void)hipFree(m->eloss);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
 }
@@ -399,12 +407,14 @@ hipError_t forward(jms_nn* m, int n) {
   return hipGetLastError();
 }
 
-// loss and gradients of the batch in act[0] / tb (n samples) into m->g
-hipError_t backward(jms_nn* m, int n) {
+// loss and gradients of the batch in act[0] / tb (n samples) into m->g; the
+// batch's mean loss is also added to *loss_acc when given
+hipError_t backward(jms_nn* m, int n, float* loss_acc = nullptr) {
   const unsigned nb = nblk((long long)n * NOUT, 256);
   float* gout = m->gbuf[NCONV + 1];
   float* gh = m->gbuf[NCONV];
-  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, m->st, m->out, m->tb, m->scale, n, gout, m->loss_part);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, m->st, m->out, m->tb, m->scale, n, gout, m->loss_part,
+                     loss_acc);
   // dense 2
   hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((long long)NOUT * (HID + 1), 256)), dim3(256), 0, m->st, gout,
                      m->act[NCONV + 1], n, HID, NOUT, m->g + OFF_D2W, m->g + OFF_D2B);
@@ -632,34 +642,52 @@ void jms_nn_data_destroy(jms_nn_data* d) {
 int jms_nn_train_epoch(jms_nn* m, jms_nn_data* d, int batch, float lr, const float* scale, unsigned int seed,
                        float* mean_loss) {
   if (!m || !d || !scale || batch < 1 || batch > m->maxb) return 1;
+  const int nbatch = d->n / batch;  // whole batches only
   std::vector<int> perm(d->n);
   for (int i = 0; i < d->n; ++i) perm[i] = i;
   std::mt19937 rng(seed);
   std::shuffle(perm.begin(), perm.end(), rng);
-  hipError_t e = hipMemcpyAsync(m->scale, scale, NOUT * 4, hipMemcpyHostToDevice, m->st);
-  double lsum = 0;
-  int nbatch = 0;
-  std::vector<float> lp;
-  for (int b0 = 0; b0 + batch <= d->n && e == hipSuccess; b0 += batch) {  // whole batches only
-    e = hipMemcpyAsync(m->idx, perm.data() + b0, batch * sizeof(int), hipMemcpyHostToDevice, m->st);
-    if (e != hipSuccess) break;
+  hipError_t e = hipSuccess;
+  if (m->perm_cap < d->n) {
+    (void)hipFree(m->perm);
+    m->perm = nullptr;
+    m->perm_cap = 0;
+    e = dalloc(&m->perm, d->n);
+    if (e == hipSuccess) m->perm_cap = d->n;
+  }
+  if (e == hipSuccess && m->eloss_cap < std::max(1, nbatch)) {
+    (void)hipFree(m->eloss);
+    m->eloss = nullptr;
+    m->eloss_cap = 0;
+    e = dalloc(&m->eloss, std::max(1, nbatch));
+    if (e == hipSuccess) m->eloss_cap = std::max(1, nbatch);
+  }
+  // the whole epoch is enqueued without host synchronisation: the
+  // permutation and the scale go up once, the batch losses come back once
+  if (e == hipSuccess) e = hipMemcpyAsync(m->perm, perm.data(), (size_t)d->n * sizeof(int), hipMemcpyHostToDevice, m->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->scale, scale, NOUT * 4, hipMemcpyHostToDevice, m->st);
+  if (e == hipSuccess && nbatch > 0) {
+    hipLaunchKernelGGL(k_zero, dim3(nblk(nbatch, 256)), dim3(256), 0, m->st, m->eloss, nbatch);
+    e = hipGetLastError();
+  }
+  for (int b = 0; b < nbatch && e == hipSuccess; ++b) {
     hipLaunchKernelGGL(k_gather, dim3(nblk((long long)batch * IN_SIZE, 256)), dim3(256), 0, m->st, d->x, d->t,
-                       m->idx, batch, m->act[0], m->tb);
+                       m->perm + (size_t)b * batch, batch, m->act[0], m->tb);
     e = forward(m, batch);
-    if (e == hipSuccess) e = backward(m, batch);
+    if (e == hipSuccess) e = backward(m, batch, m->eloss + b);
     if (e != hipSuccess) break;
     ++m->step;
     const float bc1 = 1.f - std::pow(0.9f, (float)m->step), bc2 = 1.f - std::pow(0.999f, (float)m->step);
     hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM, 256)), dim3(256), 0, m->st, m->p, m->g, m->m, m->v, m->pbf,
                        NPARAM, lr, bc1, bc2);
     e = hipGetLastError();
-    if (e == hipSuccess) {
-      const float l = sum_loss(m, batch);  // synchronises once per batch (host pointer perm stays valid)
-      lsum += l;
-      ++nbatch;
-    }
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+  std::vector<float> bl(std::max(1, nbatch), 0.f);
+  if (e == hipSuccess && nbatch > 0)
+    e = hipMemcpyAsync(bl.data(), m->eloss, (size_t)nbatch * 4, hipMemcpyDeviceToHost, m->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->st);  // perm (host) is released only after this
+  double lsum = 0;
+  for (int b = 0; b < nbatch; ++b) lsum += bl[b];
   if (mean_loss) *mean_loss = nbatch ? (float)(lsum / nbatch) : 0.f;
   return e == hipSuccess ? 0 : 2;
 }
