@@ -103,7 +103,8 @@ int jms_hip_sync(jms_hip_engine* e);
 /* integrated results of the staged scan: npos*ndetslc*ndet floats,
  * [pos][idetslc][idet] */
 int jms_hip_get_scan_results(jms_hip_engine* e, float* out);
-/* batch size in probe passes (waves resident at once); 0 = automatic */
+/* batch size in probe passes (waves resident at once, at most 65535); 0 =
+ * automatic (about 2 GiB of waves, at most 8192) */
 int jms_hip_set_batch(jms_hip_engine* e, int nwave_max);
 /* 1: capture the per-batch launch sequence in a HIP graph and replay it */
 int jms_hip_set_graph(jms_hip_engine* e, int use_graph);

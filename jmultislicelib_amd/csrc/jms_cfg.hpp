@@ -7,6 +7,7 @@
 //   PADSH: LDS padding, one complex slot per 2^PADSH elements
 #pragma once
 #include "jms_fft_device.hpp"
+#include "jms_tuned.h"
 
 namespace jms {
 
@@ -21,8 +22,8 @@ struct Cfg {
     return m;
   }
   static constexpr int T = max_bfly();                   // row pass: threads per line
-  // rows per block (six 80-thread rows at N = 480: 0.472 -> 0.456 ms per GiB)
-  static constexpr int LPB = (N == 480) ? 6 : ((256 / T) > 0 ? (256 / T) : 1);
+  // rows per block: 256 threads' worth, or the tuned value (jms_tuned.h)
+  static constexpr int LPB = Tuned<N>::LPB > 0 ? Tuned<N>::LPB : ((256 / T) > 0 ? (256 / T) : 1);
   static constexpr int C = (N <= 640) ? 16 : ((N <= 1024) ? 8 : 4);  // columns per block
   static constexpr int TC0 = 1024 / C;
   static constexpr int TC1 = (T < TC0) ? T : TC0;
@@ -35,10 +36,9 @@ struct Cfg {
   // block; used up to N = 768 (the staged tile must fit LDS)
   static constexpr int pow2_floor(int v) { int p = 1; while (2 * p <= v) p *= 2; return p; }
   static constexpr int TW = pow2_floor(T < 64 ? T : 64);
-  // 8 columns (64-byte segments) where a 16-column block cannot share its CU
-  // (per GiB of waves on MI355X: N = 360 0.893 -> 0.602 ms, N = 480
-  // 0.750 -> 0.673 ms; no gain at 640, a loss at 720)
-  static constexpr int CW = (N == 360 || N == 480) ? 8 : 16;
+  // 16 columns (128-byte segments) unless the tuner found 8 faster (where a
+  // 16-column block cannot share its CU)
+  static constexpr int CW = Tuned<N>::CW;
   static constexpr bool WL = (N <= 768) && ((TW * CW) % 64 == 0);
   // long columns (N = 1024, 2048): register-resident column kernel with 16
   // columns per block (128-byte row segments; a 2048^2 column-tile copy runs
@@ -48,10 +48,10 @@ struct Cfg {
   static constexpr int CC = XL ? 16 : (WL ? CW : C);
   static constexpr int PADSH = 3;
   // minimum waves per SIMD requested from the register allocator for the
-  // wave-local column kernel (1 = no constraint).  At N = 512 eight waves let
-  // two 1024-thread blocks share a CU, so one block's FFTs overlap the other's
-  // memory phase (measured 1.35 -> 1.11 ms per 1024 waves on MI355X).
-  static constexpr int WPE_COL = (N == 512) ? 8 : 1;
+  // wave-local column kernel (1 = no constraint; jms_tuned.h).  Eight waves
+  // let two 1024-thread blocks share a CU, so one block's FFTs overlap the
+  // other's memory phase (N = 512: 1.35 -> 1.11 ms per 1024 waves on MI355X).
+  static constexpr int WPE_COL = Tuned<N>::WPE;
   static constexpr int WPE_ROW = 1;
 
   // nontemporal wave loads/stores in the row / column pass: the streamed
@@ -66,15 +66,13 @@ struct Cfg {
   // row pass keeps the table reads
   static constexpr int SW = 0;
   static constexpr bool TWR = true;
-  // Long rows (N >= 1024): the row pass reads its twiddles through the caches
-  // from the global table (GTW_ROW) instead of an LDS copy that every block
-  // fills (16 KiB per 16 KiB row at N = 2048), with one read per butterfly
-  // and the product tree (TWR_ROW); per 1 GiB of waves on MI355X:
-  // N = 2048 0.535 -> 0.392 ms, N = 1024 0.417 -> 0.365 ms, N = 512
-  // 0.351 -> 0.348 ms (kept as is); also N = 640 0.419 -> 0.403 ms and
-  // N = 720 0.478 -> 0.448 ms
-  static constexpr bool TWR_ROW = N >= 1024 || N == 640 || N == 720;
-  static constexpr bool GTW_ROW = N >= 1024 || N == 640 || N == 720;
+  // Where the tuner chose it (jms_tuned.h; long rows): the row pass reads its
+  // twiddles through the caches from the global table (GTW_ROW) instead of
+  // an LDS copy that every block fills (16 KiB per 16 KiB row at N = 2048),
+  // with one read per butterfly and the product tree (TWR_ROW); per 1 GiB of
+  // waves on MI355X: N = 2048 0.535 -> 0.392 ms, N = 1024 0.417 -> 0.365 ms
+  static constexpr bool TWR_ROW = Tuned<N>::GTW;
+  static constexpr bool GTW_ROW = Tuned<N>::GTW;
   // fused tile I/O in the wave-local column kernel where the plan allows it
   static constexpr bool FIO = true;
 };

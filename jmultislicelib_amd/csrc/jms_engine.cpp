@@ -704,7 +704,7 @@ int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl) {
 
 int jms_hip_set_batch(jms_hip_engine* e, int nwave_max) {
   if (!e) return 1;
-  e->bcap_req = std::max(0, nwave_max);
+  e->bcap_req = std::min(std::max(0, nwave_max), 65535);  // waves are the launch grid's y dimension
   e->drop_graphs();
   return 0;
 }

@@ -1,0 +1,63 @@
+// jms_tuned.h -- launch shapes per line length chosen by the autotuner
+// (csrc/tune/tune.py) from measured kernel times on MI355X.  Generated:
+// edit by re-running the tuner.  Fields not given keep the defaults of
+// jms_cfg.hpp: LPB (rows per row-pass block; 0 = default), GTW (row-pass
+// twiddles from the global table with the product tree), CW (columns per
+// wave-local column block), WPE (waves per SIMD asked of the register
+// allocator for that kernel).
+#pragma once
+
+namespace jms {
+
+template <int N>
+struct Tuned {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+
+template <> struct Tuned<360> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 8;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<480> {
+  static constexpr int LPB = 6;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 8;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<512> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 8;
+};
+template <> struct Tuned<640> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<720> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<1024> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<2048> {
+  static constexpr int LPB = 0;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+
+}  // namespace jms

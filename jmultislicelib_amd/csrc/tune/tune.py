@@ -1,0 +1,138 @@
+#!/usr/bin/env python3
+"""Autotuner of the line-kernel launch shapes.
+
+For every supported line length it builds a timer of the row- and
+column-pass launch variants (tune_size.hip: rows per block, global twiddles,
+columns per block, waves per SIMD), runs them on the current GPU and writes
+the fastest choice per length into ../jms_tuned.h, which jms_cfg.hpp reads.
+
+  python tune.py build            # compile the timers (no GPU needed)
+  python tune.py run [--out F]    # time them on the GPU -> JSON lines
+  python tune.py emit F           # write jms_tuned.h from the JSON lines
+A variant replaces the default only when it is faster by more than 2%.
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.dirname(HERE)
+ROOT = os.path.dirname(os.path.dirname(CSRC))
+BUILD = os.path.join(CSRC, "_build", "tune")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", f"-I{ROOT}/include", f"-I{CSRC}",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
+
+
+def sizes():
+    txt = open(os.path.join(CSRC, "jms_sizes.h")).read()
+    return sorted({int(x) for x in re.findall(r"X\((\d+)\)", txt)})
+
+
+def build():
+    os.makedirs(BUILD, exist_ok=True)
+
+    def one(n):
+        out = os.path.join(BUILD, f"tune_{n}")
+        cmd = [HIPCC, *FLAGS, f"-DJMS_N={n}", os.path.join(HERE, "tune_size.hip"), "-o", out]
+        r = subprocess.run(cmd, stdin=subprocess.DEVNULL, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"tune_{n}: {r.stderr[-2000:]}")
+        return out
+
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        for p in ex.map(one, sizes()):
+            print("built", p, flush=True)
+
+
+def run(out_path):
+    with open(out_path, "w") as f:
+        for n in sizes():
+            exe = os.path.join(BUILD, f"tune_{n}")
+            r = subprocess.run(["timeout", "-k", "10", "120", exe], capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"{exe} exited {r.returncode}: {r.stdout[-500:]} {r.stderr[-500:]}")
+            f.write(r.stdout)
+            f.flush()
+            print(f"timed {n}", flush=True)
+
+
+def choose(results):
+    """{n: dict(lpb, gtw, cw, wpe)} from the timing lines; defaults unless a
+    variant is more than 2% faster"""
+    by = {}
+    for r in results:
+        by.setdefault(r["n"], []).append(r)
+    out = {}
+    for n, rs in sorted(by.items()):
+        rows = [r for r in rs if r["pass"] == "row"]
+        cols = [r for r in rs if r["pass"] == "col"]
+        k = {}
+        if rows:
+            d = next(r for r in rows if r["variant"] == "default" and r["gtw"] == 0)
+            b = min(rows, key=lambda r: r["ms"])
+            if b["ms"] < 0.98 * d["ms"]:
+                k["lpb"], k["gtw"] = b["lpb"], b["gtw"]
+        if cols:
+            d = next(r for r in cols if r["cw"] == 16 and r["wpe"] == 1)
+            b = min(cols, key=lambda r: r["ms"])
+            if b["ms"] < 0.98 * d["ms"]:
+                k["cw"], k["wpe"] = b["cw"], b["wpe"]
+        out[n] = k
+    return out
+
+
+def emit(results, path):
+    ch = choose(results)
+    lines = ["// jms_tuned.h -- launch shapes per line length chosen by the autotuner",
+             "// (csrc/tune/tune.py) from measured kernel times on MI355X.  Generated:",
+             "// edit by re-running the tuner.  Fields not given keep the defaults of",
+             "// jms_cfg.hpp: LPB (rows per row-pass block; 0 = default), GTW (row-pass",
+             "// twiddles from the global table with the product tree), CW (columns per",
+             "// wave-local column block), WPE (waves per SIMD asked of the register",
+             "// allocator for that kernel).",
+             "#pragma once", "", "namespace jms {", "",
+             "template <int N>",
+             "struct Tuned {",
+             "  static constexpr int LPB = 0;",
+             "  static constexpr bool GTW = false;",
+             "  static constexpr int CW = 16;",
+             "  static constexpr int WPE = 1;",
+             "};", ""]
+    tm = {r["n"]: r for r in results if r["pass"] == "col"}
+    for n, k in ch.items():
+        if not k:
+            continue
+        lines.append(f"template <> struct Tuned<{n}> {{")
+        lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
+        lines.append(f"  static constexpr bool GTW = {'true' if k.get('gtw') else 'false'};")
+        lines.append(f"  static constexpr int CW = {k.get('cw', 16)};")
+        lines.append(f"  static constexpr int WPE = {k.get('wpe', 1)};")
+        lines.append("};")
+    lines += ["", "}  // namespace jms", ""]
+    open(path, "w").write("\n".join(lines))
+    print(f"wrote {path}: {sum(1 for k in ch.values() if k)} tuned lengths")
+
+
+def main(argv):
+    if len(argv) < 2 or argv[1] not in ("build", "run", "emit"):
+        print(__doc__)
+        return 2
+    if argv[1] == "build":
+        build()
+    elif argv[1] == "run":
+        out = argv[argv.index("--out") + 1] if "--out" in argv else "tune_results.jsonl"
+        run(out)
+    else:
+        res = [json.loads(l) for l in open(argv[2]) if l.strip().startswith("{") and '"error"' not in l]
+        emit(res, os.path.join(CSRC, "jms_tuned.h"))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

@@ -1,0 +1,141 @@
+// tune_size.hip -- times the launch variants of the row and column passes
+// (middle slices, about 1 GiB of waves) for one line length, -DJMS_N=<n>,
+// and prints one JSON object per variant.  Used by tune.py.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+#include "jms_kernels_impl.hpp"
+
+using namespace jms;
+#ifndef JMS_N
+#error "compile with -DJMS_N=<line length>"
+#endif
+constexpr int N = JMS_N;
+using B = Cfg<N>;
+
+// row variants: rows per block (default, x2, /2) x global twiddles
+template <int LPBV, bool G>
+struct RowV : B {
+  static constexpr int LPB = LPBV;
+  static constexpr bool GTW_ROW = G;
+  static constexpr bool TWR_ROW = G;
+};
+// column variants (wave-local kernel): columns per block x waves per SIMD
+template <int CWV, int WPE>
+struct ColV : B {
+  static constexpr int CW = CWV;
+  static constexpr int CC = CWV;
+  static constexpr int WPE_COL = WPE;
+};
+
+#define CK(x)                                                                  \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      std::printf("{\"error\": \"%s\", \"line\": %d}\n", hipGetErrorString(e_), __LINE__); \
+      return 1;                                                                \
+    }                                                                          \
+  } while (0)
+
+struct Bufs {
+  float2 *w, *pro, *tw, *pg;
+  int* var;
+  int nw;
+};
+
+template <class CF>
+float time_row(const Bufs& b, int reps) {
+  dim3 g((N + CF::LPB - 1) / CF::LPB, b.nw), bl(CF::T * CF::LPB);
+  RowChan rc{};
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0);
+  for (int i = 0; i < reps; ++i)
+    hipLaunchKernelGGL((k_row_pass<N, 0, CF>), g, bl, 0, 0, b.w, N, (long long)N * N, b.pg, 0LL, (long long)N * N,
+                       b.var, 1, 0, b.tw, rc);
+  (void)hipEventRecord(e1);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  return ms / reps;
+}
+
+template <class CF>
+float time_col(const Bufs& b, int reps) {
+  ColArgs a{};
+  a.wave = b.w; a.nx = N; a.wave_stride = (long long)N * N; a.pro = b.pro; a.pro_t = b.pro; a.twg = b.tw;
+  dim3 g((N + CF::CW - 1) / CF::CW, b.nw), bl(CF::TW * CF::CW);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0);
+  for (int i = 0; i < reps; ++i) hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF>), g, bl, 0, 0, a);
+  (void)hipEventRecord(e1);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  return ms / reps;
+}
+
+template <class CF>
+void row_variant(const Bufs& b, const char* name, int lpb, bool gtw, double bytes) {
+  float best = 1e30f;
+  for (int r = 0; r < 3; ++r) best = std::fmin(best, time_row<CF>(b, 10));
+  std::printf("{\"n\": %d, \"pass\": \"row\", \"variant\": \"%s\", \"lpb\": %d, \"gtw\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n",
+              N, name, lpb, (int)gtw, best, bytes / (best * 1e-3) / 1e9);
+}
+
+template <class CF>
+void col_variant(const Bufs& b, int cw, int wpe, double bytes) {
+  float best = 1e30f;
+  for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
+  std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n", N, cw, wpe, best,
+              bytes / (best * 1e-3) / 1e9);
+}
+
+int main() {
+  Bufs b{};
+  b.nw = (int)std::min<size_t>(65535, ((size_t)1 << 30) / ((size_t)N * N * 8));  // grid y limit
+  CK(hipMalloc(&b.w, (size_t)N * N * b.nw * 8));
+  CK(hipMalloc(&b.pro, (size_t)N * N * 8));
+  CK(hipMalloc(&b.pg, (size_t)N * N * 8));
+  CK(hipMalloc(&b.tw, N * 8));
+  CK(hipMalloc(&b.var, b.nw * 4));
+  CK(hipMemset(b.w, 0, (size_t)N * N * b.nw * 8));
+  CK(hipMemset(b.var, 0, b.nw * 4));
+  std::vector<float2> h(N);
+  for (int e = 0; e < N; ++e) {
+    const double an = -2.0 * 3.14159265358979323846 * e / N;
+    h[e] = make_float2((float)std::cos(an), (float)std::sin(an));
+  }
+  CK(hipMemcpy(b.tw, h.data(), N * 8, hipMemcpyHostToDevice));
+  std::vector<float2> one((size_t)N * N, make_float2(1.f, 0.f));
+  CK(hipMemcpy(b.pro, one.data(), one.size() * 8, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b.pg, one.data(), one.size() * 8, hipMemcpyHostToDevice));
+  const double bytes = 16.0 * N * N * b.nw;
+  constexpr int L0 = (256 / B::T) > 0 ? (256 / B::T) : 1;  // default rows per block
+  constexpr int L2 = (B::T * 2 * L0 <= 1024) ? 2 * L0 : L0;
+  constexpr int LH = L0 > 1 ? L0 / 2 : 1;
+  row_variant<RowV<L0, false>>(b, "default", L0, false, bytes);
+  row_variant<RowV<L0, true>>(b, "default", L0, true, bytes);
+  if (L2 != L0) {
+    row_variant<RowV<L2, false>>(b, "x2", L2, false, bytes);
+    row_variant<RowV<L2, true>>(b, "x2", L2, true, bytes);
+  }
+  if (LH != L0) {
+    row_variant<RowV<LH, false>>(b, "half", LH, false, bytes);
+    row_variant<RowV<LH, true>>(b, "half", LH, true, bytes);
+  }
+  if constexpr (B::WL && !B::XL) {
+    col_variant<ColV<16, 1>>(b, 16, 1, bytes);
+    col_variant<ColV<16, 8>>(b, 16, 8, bytes);
+    if constexpr (B::TW % 8 == 0) {
+      col_variant<ColV<8, 1>>(b, 8, 1, bytes);
+      col_variant<ColV<8, 8>>(b, 8, 8, bytes);
+    }
+  }
+  CK(hipGetLastError());
+  return 0;
+}
