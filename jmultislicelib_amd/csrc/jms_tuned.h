@@ -17,10 +17,70 @@ struct Tuned {
   static constexpr int WPE = 1;
 };
 
+template <> struct Tuned<16> {
+  static constexpr int LPB = 32;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<48> {
+  static constexpr int LPB = 64;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<64> {
+  static constexpr int LPB = 64;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<96> {
+  static constexpr int LPB = 42;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<128> {
+  static constexpr int LPB = 16;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<160> {
+  static constexpr int LPB = 16;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<192> {
+  static constexpr int LPB = 8;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<256> {
+  static constexpr int LPB = 8;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<320> {
+  static constexpr int LPB = 8;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
   static constexpr bool GTW = false;
   static constexpr int CW = 8;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<384> {
+  static constexpr int LPB = 8;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
   static constexpr int WPE = 1;
 };
 template <> struct Tuned<480> {
@@ -30,31 +90,37 @@ template <> struct Tuned<480> {
   static constexpr int WPE = 1;
 };
 template <> struct Tuned<512> {
-  static constexpr int LPB = 0;
-  static constexpr bool GTW = false;
-  static constexpr int CW = 16;
-  static constexpr int WPE = 8;
-};
-template <> struct Tuned<640> {
-  static constexpr int LPB = 0;
+  static constexpr int LPB = 4;
   static constexpr bool GTW = true;
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
 };
+template <> struct Tuned<640> {
+  static constexpr int LPB = 3;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 8;
+  static constexpr int WPE = 1;
+};
 template <> struct Tuned<720> {
-  static constexpr int LPB = 0;
+  static constexpr int LPB = 2;
+  static constexpr bool GTW = true;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
+template <> struct Tuned<768> {
+  static constexpr int LPB = 2;
   static constexpr bool GTW = true;
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
 };
 template <> struct Tuned<1024> {
-  static constexpr int LPB = 0;
+  static constexpr int LPB = 2;
   static constexpr bool GTW = true;
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
 };
 template <> struct Tuned<2048> {
-  static constexpr int LPB = 0;
+  static constexpr int LPB = 1;
   static constexpr bool GTW = true;
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
