@@ -11,10 +11,11 @@
 
 namespace jms {
 
-// per-size launch configuration
-template <int N>
-struct Cfg {
-  using P = typename PlanFor<N>::type;
+// launch configuration of length N with FFT plan PL (the radix order; the
+// tuned plan of jms_tuned.h by default)
+template <int N, class PL = typename TunedPlan<N>::type>
+struct CfgP {
+  using P = PL;
   static_assert(plan_product<P>() == N, "plan does not multiply to N");
   static constexpr int max_bfly() {
     int m = 0;
@@ -23,7 +24,9 @@ struct Cfg {
   }
   static constexpr int T = max_bfly();                   // row pass: threads per line
   // rows per block: 256 threads' worth, or the tuned value (jms_tuned.h)
-  static constexpr int LPB = Tuned<N>::LPB > 0 ? Tuned<N>::LPB : ((256 / T) > 0 ? (256 / T) : 1);
+  // when it fits a block with this plan's threads per row
+  static constexpr int LPB = (Tuned<N>::LPB > 0 && T * Tuned<N>::LPB <= 1024) ? Tuned<N>::LPB
+                                                                               : ((256 / T) > 0 ? (256 / T) : 1);
   static constexpr int C = (N <= 640) ? 16 : ((N <= 1024) ? 8 : 4);  // columns per block
   static constexpr int TC0 = 1024 / C;
   static constexpr int TC1 = (T < TC0) ? T : TC0;
@@ -76,6 +79,10 @@ struct Cfg {
   // fused tile I/O in the wave-local column kernel where the plan allows it
   static constexpr bool FIO = true;
 };
+
+// the configuration the library is built with
+template <int N>
+struct Cfg : CfgP<N> {};
 
 
 }  // namespace jms

@@ -4,10 +4,16 @@
 // jms_cfg.hpp: LPB (rows per row-pass block; 0 = default), GTW (row-pass
 // twiddles from the global table with the product tree), CW (columns per
 // wave-local column block), WPE (waves per SIMD asked of the register
-// allocator for that kernel).
+// allocator for that kernel).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
+
+// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another
+template <int N>
+struct TunedPlan {
+  using type = typename PlanFor<N>::type;
+};
 
 template <int N>
 struct Tuned {

@@ -3,8 +3,9 @@
 
 For every supported line length it builds a timer of the row- and
 column-pass launch variants (tune_size.hip: rows per block, global twiddles,
-columns per block, waves per SIMD), runs them on the current GPU and writes
-the fastest choice per length into ../jms_tuned.h, which jms_cfg.hpp reads.
+columns per block, waves per SIMD) and of candidate FFT plans (radix orders),
+runs them on the current GPU and writes the fastest choice per length into
+../jms_tuned.h, which jms_cfg.hpp reads.
 
   python tune.py build            # compile the timers (no GPU needed)
   python tune.py run [--out F]    # time them on the GPU -> JSON lines
@@ -34,12 +35,63 @@ def sizes():
     return sorted({int(x) for x in re.findall(r"X\((\d+)\)", txt)})
 
 
+RADICES = (3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16)  # codelets of jms_fft_device.hpp
+
+
+def default_plan(n):
+    txt = open(os.path.join(CSRC, "jms_fft_device.hpp")).read()
+    m = re.search(r"PlanFor<%d>\s*\{\s*using type = Radices<([0-9, ]+)>" % n, txt)
+    return tuple(int(x) for x in m.group(1).split(","))
+
+
+def plan_candidates(n, keep=6):
+    """the default plan and the `keep` best-scored ordered factorisations of n
+    into 2 or 3 codelet radices (score: butterfly rounds, then registers per
+    thread, of a column transformed by up to 64 threads)"""
+    found = set()
+
+    def rec(rem, cur):
+        if rem == 1:
+            if 2 <= len(cur) <= 3:
+                found.add(tuple(cur))
+            return
+        if len(cur) == 3:
+            return
+        for r in RADICES:
+            if rem % r == 0:
+                rec(rem // r, cur + [r])
+
+    rec(n, [])
+
+    def score(pl):
+        t = n // min(pl)
+        tw = 1
+        while 2 * tw <= min(t, 64):
+            tw *= 2
+        rounds = sum(-(-(n // r) // tw) for r in pl)
+        nreg = max(-(-(n // r) // tw) * r for r in pl)
+        return (rounds, nreg, -pl[0])
+
+    d = default_plan(n)
+    best = sorted((p for p in found if p != d), key=score)[:keep]
+    return [d] + best
+
+
+def plans_file(n):
+    path = os.path.join(BUILD, f"plans_{n}.h")
+    items = " ".join(f'X("{",".join(map(str, p))}", Radices<{", ".join(map(str, p))}>)' for p in plan_candidates(n))
+    with open(path, "w") as f:
+        f.write(f"#define JMS_TUNE_PLANS(X) {items}\n")
+    return path
+
+
 def build():
     os.makedirs(BUILD, exist_ok=True)
 
     def one(n):
         out = os.path.join(BUILD, f"tune_{n}")
-        cmd = [HIPCC, *FLAGS, f"-DJMS_N={n}", os.path.join(HERE, "tune_size.hip"), "-o", out]
+        cmd = [HIPCC, *FLAGS, f"-DJMS_N={n}", f'-DJMS_PLANS_FILE="{plans_file(n)}"',
+               os.path.join(HERE, "tune_size.hip"), "-o", out]
         r = subprocess.run(cmd, stdin=subprocess.DEVNULL, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             raise RuntimeError(f"tune_{n}: {r.stderr[-2000:]}")
@@ -83,6 +135,13 @@ def choose(results):
             b = min(cols, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
                 k["cw"], k["wpe"] = b["cw"], b["wpe"]
+        plans = [r for r in rs if r["pass"] == "plan"]
+        if plans:
+            dname = ",".join(map(str, default_plan(n)))
+            d = next((r for r in plans if r["plan"] == dname), None)
+            b = min(plans, key=lambda r: r["row_ms"] + r["col_ms"])
+            if d and b["plan"] != dname and b["row_ms"] + b["col_ms"] < 0.98 * (d["row_ms"] + d["col_ms"]):
+                k["plan"] = b["plan"]
         out[n] = k
     return out
 
@@ -95,8 +154,13 @@ def emit(results, path):
              "// jms_cfg.hpp: LPB (rows per row-pass block; 0 = default), GTW (row-pass",
              "// twiddles from the global table with the product tree), CW (columns per",
              "// wave-local column block), WPE (waves per SIMD asked of the register",
-             "// allocator for that kernel).",
+             "// allocator for that kernel).  Included by jms_cfg.hpp after the plans.",
              "#pragma once", "", "namespace jms {", "",
+             "// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another",
+             "template <int N>",
+             "struct TunedPlan {",
+             "  using type = typename PlanFor<N>::type;",
+             "};", "",
              "template <int N>",
              "struct Tuned {",
              "  static constexpr int LPB = 0;",
@@ -104,9 +168,11 @@ def emit(results, path):
              "  static constexpr int CW = 16;",
              "  static constexpr int WPE = 1;",
              "};", ""]
-    tm = {r["n"]: r for r in results if r["pass"] == "col"}
     for n, k in ch.items():
-        if not k:
+        if "plan" in k:
+            lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
+    for n, k in ch.items():
+        if not {"lpb", "gtw", "cw", "wpe"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
@@ -116,7 +182,8 @@ def emit(results, path):
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))
-    print(f"wrote {path}: {sum(1 for k in ch.values() if k)} tuned lengths")
+    print(f"wrote {path}: {sum(1 for k in ch.values() if k)} tuned lengths, "
+          f"{sum(1 for k in ch.values() if 'plan' in k)} with another plan")
 
 
 def main(argv):

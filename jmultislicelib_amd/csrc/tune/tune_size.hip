@@ -62,16 +62,26 @@ float time_row(const Bufs& b, int reps) {
   return ms / reps;
 }
 
+// column pass of a middle slice with the kernel the launcher would pick for CF
 template <class CF>
 float time_col(const Bufs& b, int reps) {
   ColArgs a{};
   a.wave = b.w; a.nx = N; a.wave_stride = (long long)N * N; a.pro = b.pro; a.pro_t = b.pro; a.twg = b.tw;
-  dim3 g((N + CF::CW - 1) / CF::CW, b.nw), bl(CF::TW * CF::CW);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   (void)hipEventRecord(e0);
-  for (int i = 0; i < reps; ++i) hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF>), g, bl, 0, 0, a);
+  for (int i = 0; i < reps; ++i) {
+    if constexpr (CF::XL) {
+      hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF>), dim3((N + 15) / 16, b.nw), dim3(1024), 0, 0, a);
+    } else if constexpr (CF::WL) {
+      hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF>), dim3((N + CF::CW - 1) / CF::CW, b.nw),
+                         dim3(CF::TW * CF::CW), 0, 0, a);
+    } else {
+      hipLaunchKernelGGL((k_col_pass<N, false, false, CF>), dim3((N + CF::C - 1) / CF::C, b.nw),
+                         dim3(CF::TC * CF::C), 0, 0, a);
+    }
+  }
   (void)hipEventRecord(e1);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -93,6 +103,24 @@ void col_variant(const Bufs& b, int cw, int wpe, double bytes) {
   for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
   std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n", N, cw, wpe, best,
               bytes / (best * 1e-3) / 1e9);
+}
+
+// FFT plan candidates (a file written by tune.py: JMS_TUNE_PLANS(X) lists
+// X("r0,r1,...", Radices<...>)), each timed with the tuned launch knobs
+#ifdef JMS_PLANS_FILE
+#include JMS_PLANS_FILE
+#endif
+
+template <class PL>
+void plan_variant(const Bufs& b, const char* name, double bytes) {
+  using C = CfgP<N, PL>;
+  float br = 1e30f, bc = 1e30f;
+  for (int r = 0; r < 3; ++r) {
+    br = std::fmin(br, time_row<C>(b, 10));
+    bc = std::fmin(bc, time_col<C>(b, 10));
+  }
+  std::printf("{\"n\": %d, \"pass\": \"plan\", \"plan\": \"%s\", \"row_ms\": %.5f, \"col_ms\": %.5f, "
+              "\"gbs\": %.1f}\n", N, name, br, bc, 2.0 * bytes / ((br + bc) * 1e-3) / 1e9);
 }
 
 int main() {
@@ -136,6 +164,11 @@ int main() {
       col_variant<ColV<8, 8>>(b, 8, 8, bytes);
     }
   }
+#ifdef JMS_PLANS_FILE
+#define JMS_PLAN_CASE(NAME, ...) plan_variant<__VA_ARGS__>(b, NAME, bytes);
+  JMS_TUNE_PLANS(JMS_PLAN_CASE)
+#undef JMS_PLAN_CASE
+#endif
   CK(hipGetLastError());
   return 0;
 }
