@@ -23,6 +23,10 @@ struct Tuned {
   static constexpr int WPE = 1;
 };
 
+template <> struct TunedPlan<32> { using type = Radices<4, 8>; };
+template <> struct TunedPlan<48> { using type = Radices<6, 8>; };
+template <> struct TunedPlan<128> { using type = Radices<4, 8, 4>; };
+template <> struct TunedPlan<320> { using type = Radices<5, 8, 8>; };
 template <> struct Tuned<16> {
   static constexpr int LPB = 32;
   static constexpr bool GTW = true;

@@ -21,7 +21,7 @@ TOL = 1e-4
 
 
 @pytest.mark.parametrize("ny,nx", [(64, 64), (256, 256), (512, 512), (360, 384), (128, 256),
-                                   (720, 640), (96, 480), (2048, 64), (16, 768)])
+                                   (720, 640), (96, 480), (2048, 64), (16, 768), (320, 32), (48, 128)])
 def test_fft2d_vs_oracle(ny, nx):
     rng = np.random.default_rng(nx * 7 + ny)
     a = (rng.standard_normal((2, ny, nx)) + 1j * rng.standard_normal((2, ny, nx))).astype(np.complex64)
@@ -139,7 +139,8 @@ def test_staged_scan_vs_oracle(graph):
 
 @pytest.mark.parametrize("nx,ny", [(128, 64), (64, 128), (360, 384), (96, 160), (2048, 64), (64, 2048),
                                    (1024, 96), (96, 1024), (768, 640), (360, 1024), (48, 2048),
-                                   (384, 360), (96, 480), (480, 720), (720, 64)])
+                                   (384, 360), (96, 480), (480, 720), (720, 64), (320, 48), (48, 320),
+                                   (32, 128)])
 def test_nonsquare_mixed_radix(nx, ny):
     sp = random_setup(nx, ny, nslices=6, nvar=(3, 2), ndetper=2, seed=nx + ny)
     var = configs.variant_sequences(sp, 2, seed=3)
