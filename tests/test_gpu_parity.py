@@ -174,3 +174,24 @@ def test_single_probe_256_vs_oracle_fp32_fp64():
           f"oracle f32 {rel_l2(r32, r64):.2e}/{rel_l2(w32, w64):.2e}")
     print("detectors gpu", g_int.ravel(), "f64", r64.ravel())
     assert e_int < TOL and e_wav < TOL
+
+
+def test_full_2048_grid_batch_vs_oracle():
+    """2048^2 (the long-line kernels on a full grid): a staged batch of two
+    probe passes with detection planes every 2 slices, against the oracle."""
+    sp = random_setup(2048, 2048, nslices=5, nvar=(2, 2), ndetper=2, seed=21)
+    npos = 2
+    dx = np.array([0.3, 1.1], np.float32)
+    dy = np.array([0.7, 0.2], np.float32)
+    var = configs.variant_sequences(sp, npos, seed=4)
+    eng = Engine().setup(sp)
+    eng.set_batch(2)
+    eng.stage_scan(dx, dy, None, np.ones(1, np.float32), var)
+    eng.run_staged(0, npos)
+    eng.sync()
+    g = eng.scan_results()
+    ref = O.stem_scan(sp, dx, dy, np.ones(1, np.float32), var, prec=32, nthreads=2)
+    e = rel_l2(g, ref)
+    print(f"2048^2 batch: rel L2 {e:.2e}")
+    assert e < TOL
+    eng.close()
