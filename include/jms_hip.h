@@ -59,6 +59,27 @@ int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* 
  * gratings from its caller. */
 int jms_hip_gaussian_gratings(jms_hip_engine* e, int islc, int nvar, const int* nat, const double* xy,
                               const int* z, double sigma, double sig_int);
+/* generate nvar variants of structure slice islc on the device from
+ * projected atomic form factors, in place of an upload -- the device path of
+ * CJEElaSca::CalculatePhasegratingGPU (JEElaSca.cpp:978-1159, kernels of
+ * scapot.cu:29-89) for neutral atoms:
+ *   V(q) = sum_t ff_t(q) ap(q) sum_{atoms of t} occ exp(-2 pi i (x qx + y qy)),
+ *   V(r) = IFT[V(q)] * POT_SCA / (a0x a0y cz),
+ *   t(r) = exp(i PGR_SIG cz wl V(r))  (complex V: the imaginary part absorbs),
+ * then with JMS_HIP_FF_BWL_PGR t = IFT[FT[t] * (|q| <= 2/3 kmax) / n].
+ * ff2d: ntype 2-D form factors [nm] (2*nx*ny floats each, FFT order, the
+ * reference's m_h_att_ff2d, including any Debye-Waller and absorptive
+ * parts); per variant v: nat[v] atoms, positions xy (2 floats per atom [nm]
+ * in the grid cell), type index and occupancy (occ NULL = 1), concatenated
+ * over the variants; cz: slice thickness [nm]; flags: JMS_HIP_FF_BWL_POT
+ * (potential band limit 1/2 - 1/2 tanh(30 (|q|/kmax - 0.9))),
+ * JMS_HIP_FF_BWL_PGR (phase-grating band limit, the reference's default).
+ * Atom displacements (QEP draws) are the caller's: positions are final. */
+#define JMS_HIP_FF_BWL_POT 1
+#define JMS_HIP_FF_BWL_PGR 2
+int jms_hip_formfactor_gratings(jms_hip_engine* e, int islc, int nvar, int ntype, const float* ff2d,
+                                const int* nat, const float* xy, const int* type, const float* occ, float cz,
+                                int flags);
 /* copy nvar variants of structure slice islc (2*nx*ny floats each) */
 int jms_hip_get_gratings(jms_hip_engine* e, int islc, int nvar, float* out);
 /* npro propagators, proidx[nslc] selects one per structure slice -- :542 */

@@ -94,6 +94,8 @@ _SIGS = [
     ("jms_hip_upload_gratings", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("jms_hip_gaussian_gratings", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_double,
                                           c_double]),
+    ("jms_hip_formfactor_gratings", c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_float, c_int]),
     ("jms_hip_get_gratings", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("jms_hip_setup_propagators", c_int, [c_void_p, c_int, c_int, c_void_p]),
     ("jms_hip_upload_propagator", c_int, [c_void_p, c_int, c_void_p]),

@@ -511,6 +511,65 @@ int jms_hip_gaussian_gratings(jms_hip_engine* e, int islc, int nvar, const int* 
   return 0;
 }
 
+int jms_hip_formfactor_gratings(jms_hip_engine* e, int islc, int nvar, int ntype, const float* ff2d,
+                                const int* nat, const float* xy, const int* type, const float* occ, float cz,
+                                int flags) {
+  if (!e) return 1;
+  if (islc < 0 || islc >= e->nscslc || nvar < 1 || ntype < 1 || !ff2d || !nat || !(cz > 0.f))
+    return e->fail(2, "invalid grating setup");
+  if (e->npgx != e->nx || e->npgy != e->ny) return e->fail(3, "gratings must match the grid");
+  const int nl = std::min(nvar, e->nvarslc[islc]);
+  const int nx = e->nx, ny = e->ny;
+  const size_t n = (size_t)nx * ny;
+  // atoms of each set grouped by type, ascending atom order inside a type
+  // (the reference's loop order: types outer, atoms inner, JEElaSca.cpp:1088-1122)
+  std::vector<jms::FFAtom> atoms;
+  std::vector<int> off(nl + 1, 0);
+  size_t a_base = 0;
+  for (int v = 0; v < nl; ++v) {
+    if (nat[v] < 0 || (nat[v] > 0 && (!xy || !type))) return e->fail(4, "invalid atom list");
+    for (int at = 0; at < nat[v]; ++at) {
+      const int t = type[a_base + at];
+      if (t < 0 || t >= ntype) return e->fail(4, "atom type out of range");
+    }
+    for (int t = 0; t < ntype; ++t)
+      for (int at = 0; at < nat[v]; ++at) {
+        const size_t k = a_base + at;
+        if (type[k] != t) continue;
+        atoms.push_back(jms::FFAtom{xy[2 * k], xy[2 * k + 1], occ ? occ[k] : 1.f, t});
+      }
+    off[v + 1] = (int)atoms.size();
+    a_base += nat[v];
+  }
+  if (atoms.empty()) atoms.push_back(jms::FFAtom{0.f, 0.f, 0.f, 0});
+  // JEElaSca.h:88,91,94: potential scale [V nm^2 per form-factor nm] and
+  // interaction constant (their product is 1 / (4 pi)), band-limit fraction
+  constexpr float POT_SCA = 0.03809982f, PGR_SIG = 2.08865737f, PGR_BWL = 0.6666667f;
+  const float itogx = 1.f / e->a0x, itogy = 1.f / e->a0y;
+  const float kmax = std::min((float)(nx / 2) * itogx, (float)(ny / 2) * itogy);
+  const float vol = e->a0x * e->a0y * cz;
+  const float sc = POT_SCA / vol, sg = PGR_SIG * cz * e->wl;
+  const float ftrn = 1.f / (float)nx / (float)ny;
+  (void)hipSetDevice(e->device);
+  DevBuf d_ff, d_atoms, d_off;
+  hipError_t err = d_ff.ensure(sizeof(float) * 2 * n * ntype);
+  if (err == hipSuccess) err = d_atoms.ensure(sizeof(jms::FFAtom) * atoms.size());
+  if (err == hipSuccess) err = d_off.ensure(sizeof(int) * off.size());
+  if (err == hipSuccess) err = hipMemcpy(d_ff.p, ff2d, sizeof(float) * 2 * n * ntype, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(d_atoms.p, atoms.data(), sizeof(jms::FFAtom) * atoms.size(), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = jms::launch_ff_gratings(e->pgr.as<float2>() + e->slcoffset[islc], nx, ny, nl, d_ff.as<float2>(),
+                                  (const jms::FFAtom*)d_atoms.p, d_off.as<int>(), itogx, itogy, kmax,
+                                  (flags & JMS_HIP_FF_BWL_POT) ? 1 : 0, sc, sg, (flags & JMS_HIP_FF_BWL_PGR) ? 1 : 0,
+                                  kmax * PGR_BWL, ftrn, e->twx.as<float2>(), e->twy.as<float2>(), e->st);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+  d_ff.release(); d_atoms.release(); d_off.release();
+  if (err != hipSuccess) return e->fail(102, "form-factor phase gratings on the device", err);
+  return 0;
+}
+
 int jms_hip_get_gratings(jms_hip_engine* e, int islc, int nvar, float* out) {
   if (!e || !out) return 1;
   if (islc < 0 || islc >= e->nscslc || nvar < 1) return e->fail(2, "invalid grating read");

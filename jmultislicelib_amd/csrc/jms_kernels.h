@@ -72,6 +72,13 @@ struct GaussAtom {
   int i0, i1, j0, j1;
 };
 
+// one atom of the form-factor model: position [nm] in the slice cell,
+// occupancy, atom type (index of its 2-D form factor)
+struct FFAtom {
+  float x, y, occ;
+  int type;
+};
+
 constexpr int kMaxDetPerSweep = 8;
 
 bool size_supported(int n);
@@ -104,6 +111,14 @@ hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, h
 hipError_t launch_gauss_gratings(float2* out, int nx, int ny, int nvar, double hx, double hy,
                                  double rc2, double inv2s2, double sig_int, const GaussAtom* atoms,
                                  const int* off, const int* idx, hipStream_t st);
+// phase gratings of nvar atom sets from projected form factors into out +
+// v * n: structure-factor sum (ff[t][n] complex per atom type, atoms of set v
+// at off[v] .. off[v+1] grouped by type), inverse 2-D transform, scale sc and
+// exp(i sg V), and with bwl_pgr the band limit |q| <= kthr (weight ftrn)
+hipError_t launch_ff_gratings(float2* out, int nx, int ny, int nvar, const float2* ff, const FFAtom* atoms,
+                              const int* off, float itogx, float itogy, float kmax, int bwl_pot, float sc,
+                              float sg, int bwl_pgr, float kthr, float ftrn, const float2* twx,
+                              const float2* twy, hipStream_t st);
 // probe into out (n complex), partial: ceil(n/256) doubles of scratch
 hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st);
 

@@ -208,6 +208,106 @@ __global__ __launch_bounds__(256) void k_gauss_gratings(float2* __restrict__ out
   out[(long long)v * nx * ny + (long long)py * nx + px] = make_float2((float)cos(ph), (float)sin(ph));
 }
 
+// ---------------------------------------------------------------------------
+// phase gratings from projected atomic form factors (CalculatePhasegratingGPU,
+// JEElaSca.cpp:978-1159; AddFormFactor2dKernel / PotToPgrKernel,
+// scapot.cu:29-41,80-89).  Spatial frequencies in FFT order [1/nm],
+// q = fidx(i) * (1 / a) (InitHelper, JEElaSca.cpp:529-541); every product
+// and sum that decides an aperture edge is rounded as written, so that the
+// edge pixels do not depend on contraction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ff_freq(int i, int n, float itog) { return __fmul_rn((float)fidx(i, n), itog); }
+__device__ __forceinline__ float ff_gmag(float qx, float qy) {
+  return __fsqrt_rn(__fadd_rn(__fmul_rn(qx, qx), __fmul_rn(qy, qy)));
+}
+
+// Fourier-space projected potential of atom set v (atoms off[v] .. off[v+1],
+// grouped by type, ascending atom order inside a type):
+//   pot(q) = sum_t ff_t(q) ap(q) sum_{atoms of t} occ exp(-2 pi i (x qx + y qy))
+// ap(q) = 1/2 - 1/2 tanh(30 (|q|/kmax - 0.9)) with the potential band limit,
+// else 1 (JEElaSca.cpp:592-595).  The atoms pass through LDS 256 at a time;
+// every thread walks the same list, so the type switches are uniform.
+// grid: (ceil(nx*ny/256), nvar), block: 256
+__global__ __launch_bounds__(256) void k_ff_sum(float2* __restrict__ out, const float2* __restrict__ ff,
+                                                const FFAtom* __restrict__ atoms, const int* __restrict__ off,
+                                                int nx, int ny, float itogx, float itogy, float kmax,
+                                                int bwl_pot) {
+  __shared__ FFAtom s_at[256];
+  const long long n = (long long)nx * ny;
+  const int v = blockIdx.y;
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = p < n;
+  const int ix = valid ? (int)(p % nx) : 0, iy = valid ? (int)(p / nx) : 0;
+  const float qx = ff_freq(ix, nx, itogx), qy = ff_freq(iy, ny, itogy);
+  float ap = 1.f;
+  if (bwl_pot) ap = 0.5f - 0.5f * tanhf((ff_gmag(qx, qy) / kmax - 0.9f) * 30.f);
+  const int a0 = off[v], a1 = off[v + 1];
+  float2 pot = make_float2(0.f, 0.f), s = make_float2(0.f, 0.f);
+  int cur = -1;  // type of the running sum s
+  for (int c0 = a0; c0 < a1; c0 += 256) {
+    const int m = (a1 - c0 < 256) ? (a1 - c0) : 256;
+    __syncthreads();  // the previous chunk's reads are done
+    if ((int)threadIdx.x < m) s_at[threadIdx.x] = atoms[c0 + threadIdx.x];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+      const FFAtom at = s_at[k];
+      if (at.type != cur) {
+        if (cur >= 0 && valid) {
+          const float2 f = ff[(long long)cur * n + p];
+          pot.x += ap * (f.x * s.x - f.y * s.y);
+          pot.y += ap * (f.x * s.y + f.y * s.x);
+        }
+        s = make_float2(0.f, 0.f);
+        cur = at.type;
+      }
+      // phase in turns, formed in double (x q reaches ~100 turns) and
+      // reduced before the float sine and cosine
+      double ud = (double)at.x * (double)qx + (double)at.y * (double)qy;
+      ud -= rint(ud);
+      float sn, cs;
+      sincospif(2.f * (float)ud, &sn, &cs);
+      s.x += at.occ * cs;
+      s.y -= at.occ * sn;
+    }
+  }
+  if (!valid) return;
+  if (cur >= 0) {
+    const float2 f = ff[(long long)cur * n + p];
+    pot.x += ap * (f.x * s.x - f.y * s.y);
+    pot.y += ap * (f.x * s.y + f.y * s.x);
+  }
+  out[(long long)v * n + p] = pot;
+}
+
+// a <- exp(i sg sc a): real-space potential (complex: the imaginary part
+// absorbs) to transmission function, exp(-sg Im V) (cos sg Re V, sin sg Re V)
+// with V = sc a (JEElaSca.cpp:1129-1141; PotToPgrKernel)
+__global__ void k_pot_pgr(float2* __restrict__ a, long long n, float sc, float sg) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 p = a[i];
+  const float vr = p.x * sc, vi = p.y * sc;
+  const float m = expf(-sg * vi);
+  float sn, cs;
+  sincosf(sg * vr, &sn, &cs);
+  a[i] = make_float2(m * cs, m * sn);
+}
+
+// a(q) <- a(q) * (|q| > kthr ? 0 : ftrn) on nvar Fourier-space gratings
+// (phase-grating band limit, JEElaSca.cpp:597-604 with ftrn = 1/(nx ny))
+__global__ void k_pgr_aperture(float2* __restrict__ a, int nx, int ny, long long total, float itogx,
+                               float itogy, float kthr, float ftrn) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long p = i % ((long long)nx * ny);
+  const int ix = (int)(p % nx), iy = (int)(p / nx);
+  const float w = (ff_gmag(ff_freq(ix, nx, itogx), ff_freq(iy, ny, itogy)) > kthr) ? 0.f : ftrn;
+  float2 z = a[i];
+  z.x *= w;
+  z.y *= w;
+  a[i] = z;
+}
+
 // dst += w * src (complex), n items
 __global__ void k_axpy_c(float2* __restrict__ dst, const float2* __restrict__ src, long long n, float w) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -330,6 +430,28 @@ hipError_t launch_gauss_gratings(float2* out, int nx, int ny, int nvar, double h
   hipLaunchKernelGGL(k_gauss_gratings, g, b, 0, st, out, nx, ny, hx, hy, rc2, inv2s2, sig_int, atoms, off,
                      idx);
   return hipGetLastError();
+}
+
+hipError_t launch_ff_gratings(float2* out, int nx, int ny, int nvar, const float2* ff, const FFAtom* atoms,
+                              const int* off, float itogx, float itogy, float kmax, int bwl_pot, float sc,
+                              float sg, int bwl_pgr, float kthr, float ftrn, const float2* twx,
+                              const float2* twy, hipStream_t st) {
+  const long long n = (long long)nx * ny, total = n * nvar;
+  hipLaunchKernelGGL(k_ff_sum, dim3((unsigned)((n + 255) / 256), nvar), dim3(256), 0, st, out, ff, atoms, off, nx,
+                     ny, itogx, itogy, kmax, bwl_pot);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = launch_fft2d(nx, ny, nvar, out, +1, twx, twy, st);
+  if (e != hipSuccess) return e;
+  const dim3 g((unsigned)((total + 255) / 256)), b(256);
+  hipLaunchKernelGGL(k_pot_pgr, g, b, 0, st, out, total, sc, sg);
+  e = hipGetLastError();
+  if (e != hipSuccess || !bwl_pgr) return e;
+  e = launch_fft2d(nx, ny, nvar, out, -1, twx, twy, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pgr_aperture, g, b, 0, st, out, nx, ny, total, itogx, itogy, kthr, ftrn);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_fft2d(nx, ny, nvar, out, +1, twx, twy, st);
 }
 
 hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st) {

@@ -331,6 +331,24 @@ class Engine:
         self._c("jms_hip_gaussian_gratings", islc, len(sites), ptr(nat), ptr(xy), ptr(z), float(sigma),
                 float(sig_int))
 
+    def formfactor_gratings(self, islc, ff2d, sites, cz, bwl_pot=False, bwl_pgr=True):
+        """Generate the variants of structure slice islc on the device from
+        projected atomic form factors ff2d (ntype, ny, nx) [nm] (FFT order)
+        and per-variant atom lists [(xy [nm] (k, 2), type (k,), occ (k,) or
+        None), ...]; cz = slice thickness [nm] (the device path of
+        CJEElaSca::CalculatePhasegratingGPU, JEElaSca.cpp:978-1159)."""
+        ff = np.ascontiguousarray(ff2d, np.complex64)
+        if ff.ndim != 3 or ff.shape[1:] != (self.sp.ny, self.sp.nx):
+            raise ValueError(f"ff2d must be (ntype, {self.sp.ny}, {self.sp.nx})")
+        nat = _i32([len(t) for _, t, _ in sites])
+        xy = _f32(np.concatenate([np.asarray(x, np.float32).reshape(-1, 2) for x, _, _ in sites]).reshape(-1))
+        typ = _i32(np.concatenate([np.asarray(t).reshape(-1) for _, t, _ in sites]))
+        occ = _f32(np.concatenate([np.ones(len(t), np.float32) if o is None else np.asarray(o, np.float32)
+                                   for _, t, o in sites]))
+        flags = (1 if bwl_pot else 0) | (2 if bwl_pgr else 0)
+        self._c("jms_hip_formfactor_gratings", islc, len(sites), ff.shape[0], ptr(ff), ptr(nat), ptr(xy), ptr(typ),
+                ptr(occ), float(cz), flags)
+
     def gratings(self, islc, nvar) -> np.ndarray:
         out = np.zeros((nvar, self.sp.ny, self.sp.nx), np.complex64)
         self._c("jms_hip_get_gratings", islc, nvar, ptr(out))

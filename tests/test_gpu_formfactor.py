@@ -1,0 +1,130 @@
+"""Phase gratings from projected atomic form factors on the device
+(jms_hip_formfactor_gratings, the device path of
+CJEElaSca::CalculatePhasegratingGPU, JEElaSca.cpp:978-1159) against the
+oracle (oracle/elasca_ref.py, float64), rel. L2 < 1e-4 on the complex64
+gratings, for every combination of the two band limits on a non-square grid
+with two absorptive atom types, partial occupancies and atoms outside the
+cell; one Gaussian atom against its closed form directly; and a multislice
+run on the device gratings against the oracle run on the oracle's
+gratings."""
+import numpy as np
+import pytest
+
+from jmultislicelib_amd import Engine, configs
+from jmultislicelib_amd.jms import JMultiSlice
+from oracle import elasca_ref as E
+from oracle import oracle as O
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, AX, AY, CZ = 128, 96, 1.6, 1.2, 0.2
+
+
+def ff_setup(nvar, nslices=6):
+    J = JMultiSlice()
+    J.SetHighTension(200.0)
+    J.SetGridSize(NX, NY)
+    J.SetSupercellSize(AX, AY, CZ)
+    wl = J.GetWaveLength()
+    pro = J.CalculatePropagator(CZ, 0.0, 0.0, 1)
+    dets = [J.CalculateRingDetector(b0, b1) for (b0, b1) in configs.DETECTOR_RANGES]
+    probe = J.CalculateProbeWaveFourier(25.0, 0.03)
+    objslc = np.zeros(nslices, np.int32)
+    det_objslc, ndetslc = configs.detection_planes(0, nslices)
+    return configs.Setup(nx=NX, ny=NY, a0=(AX, AY, CZ), ht=200.0, wl=wl, nvarslc=[nvar],
+                         gratings=[np.ones((nvar, NY, NX), np.complex64)], propagators=[pro], proidx=[0],
+                         objslc=objslc, det_objslc=det_objslc, ndetslc=ndetslc, detectors=dets, probe=probe,
+                         slice_thickness=CZ)
+
+
+def form_factors():
+    """two types: Gaussian form factors with a 5% / 8% absorptive part; the
+    first is sharp enough (s = 0.012 nm) to reach past 0.9 kmax, where the
+    potential band limit acts"""
+    qx, qy = E.freqs(NX, AX).astype(np.float64), E.freqs(NY, AY).astype(np.float64)
+    q2 = qx[None, :] ** 2 + qy[:, None] ** 2
+    out = []
+    for s, amp, ab in ((0.012, 25.0, 0.05), (0.08, 70.0, 0.08)):
+        out.append(amp * np.exp(-2 * np.pi ** 2 * s * s * q2) * (1 + 1j * ab))
+    return np.stack(out).astype(np.complex64)
+
+
+def atom_sets(nvar, nat, seed):
+    rng = np.random.default_rng(seed)
+    sets = []
+    for _ in range(nvar):
+        xy = np.stack([rng.uniform(-0.3, AX + 0.3, nat), rng.uniform(-0.3, AY + 0.3, nat)], 1).astype(np.float32)
+        typ = rng.integers(0, 2, nat).astype(np.int32)
+        occ = rng.uniform(0.5, 1.0, nat).astype(np.float32)
+        sets.append((xy, typ, occ))
+    return sets
+
+
+@pytest.mark.parametrize("bwl_pot,bwl_pgr", [(False, False), (False, True), (True, False), (True, True)])
+def test_formfactor_gratings_vs_oracle(bwl_pot, bwl_pgr):
+    nvar = 3
+    sp = ff_setup(nvar)
+    eng = Engine().setup(sp)
+    ff = form_factors()
+    sets = atom_sets(nvar, 40, seed=11)
+    eng.formfactor_gratings(0, ff, sets, CZ, bwl_pot=bwl_pot, bwl_pgr=bwl_pgr)
+    dev = eng.gratings(0, nvar)
+    for v, (xy, typ, occ) in enumerate(sets):
+        ref = E.phase_grating(NX, NY, AX, AY, CZ, sp.wl, ff, xy, typ, occ, bwl_pot=bwl_pot, bwl_pgr=bwl_pgr)
+        ph = np.abs(np.angle(ref)).max()
+        e = rel_l2(dev[v], ref)
+        print(f"bwl_pot {bwl_pot} bwl_pgr {bwl_pgr} variant {v}: rel L2 {e:.2e}, max |phase| {ph:.2f} rad")
+        assert ph > 0.3  # a strong object
+        assert e < 1e-4
+        other = E.phase_grating(NX, NY, AX, AY, CZ, sp.wl, ff, xy, typ, occ, bwl_pot=not bwl_pot, bwl_pgr=bwl_pgr)
+        assert rel_l2(other, ref) > 1e-3  # the potential band limit matters here
+    eng.close()
+
+
+def test_gaussian_atom_closed_form():
+    """one Gaussian atom, no band limit: phase wl/(4 pi) * amp * periodic
+    Gaussian, |t| = 1 (independent of the oracle)"""
+    sp = ff_setup(1)
+    eng = Engine().setup(sp)
+    s, amp, x0, y0 = 0.07, 150.0, 1.37, 0.21
+    qx, qy = E.freqs(NX, AX).astype(np.float64), E.freqs(NY, AY).astype(np.float64)
+    ff = (amp * np.exp(-2 * np.pi ** 2 * s * s * (qx[None, :] ** 2 + qy[:, None] ** 2)))[None].astype(np.complex64)
+    eng.formfactor_gratings(0, ff, [(np.array([[x0, y0]], np.float32), np.zeros(1, np.int32), None)], CZ,
+                            bwl_pgr=False)
+    t = eng.gratings(0, 1)[0]
+    x, y = np.arange(NX) * AX / NX, np.arange(NY) * AY / NY
+    g = np.zeros((NY, NX))
+    for mx in (-1, 0, 1):
+        for my in (-1, 0, 1):
+            d2 = (x[None, :] - x0 - mx * AX) ** 2 + (y[:, None] - y0 - my * AY) ** 2
+            g += np.exp(-d2 / (2 * s * s)) / (2 * np.pi * s * s)
+    ph = sp.wl / (4 * np.pi) * amp * g
+    print(f"max phase {ph.max():.3f} rad; max |angle - closed form| {np.abs(np.angle(t) - ph).max():.2e}")
+    assert ph.max() > 0.5
+    assert np.abs(np.abs(t) - 1).max() < 1e-5
+    assert np.abs(np.angle(t) - ph).max() < 2e-5
+    eng.close()
+
+
+def test_multislice_on_formfactor_gratings_vs_oracle():
+    nvar = 2
+    sp = ff_setup(nvar, nslices=8)
+    eng = Engine().setup(sp)
+    ff = form_factors()
+    sets = atom_sets(nvar, 30, seed=12)
+    eng.formfactor_gratings(0, ff, sets, CZ)
+    ref_pgr = np.stack([E.phase_grating(NX, NY, AX, AY, CZ, sp.wl, ff, xy, t, o) for xy, t, o in sets])
+    sp_ref = ff_setup(nvar, nslices=8)
+    sp_ref.gratings = [ref_pgr.astype(np.complex64)]
+    var = np.array([0, 1, 1, 0, 1, 0, 0, 1], np.int32)
+    eng.wave_offset(0.31, 0.44)
+    eng.multislice(var)
+    got = eng.result()
+    win = O.offset_wave(sp.probe, AX, AY, sp.wl, 0.31, 0.44)
+    want, _ = O.multislice(sp_ref, win, var)
+    e = rel_l2(got, want)
+    print(f"detectors device {got.ravel()} oracle {want.ravel()}: rel L2 {e:.2e}")
+    assert e < 1e-4
+    eng.close()
