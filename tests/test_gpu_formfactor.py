@@ -128,3 +128,48 @@ def test_multislice_on_formfactor_gratings_vs_oracle():
     print(f"detectors device {got.ravel()} oracle {want.ravel()}: rel L2 {e:.2e}")
     assert e < 1e-4
     eng.close()
+
+
+def test_full_size_perovskite_variants():
+    """512^2 over 8 x 8 SrTiO3 cells (3.124 nm), both slice types (SrO: 128
+    atoms, TiO2: 192 atoms), 8 frozen-phonon variants each (0.008 nm rms
+    displacements), three absorptive species, reference band limits"""
+    n, a = 512, 8 * 0.3905
+    J = JMultiSlice()
+    J.SetHighTension(200.0)
+    J.SetGridSize(n, n)
+    J.SetSupercellSize(a, a, 0.19525)
+    wl = J.GetWaveLength()
+    nvar = 8
+    sp = configs.Setup(nx=n, ny=n, a0=(a, a, 0.19525), ht=200.0, wl=wl, nvarslc=[nvar, nvar],
+                       gratings=[np.ones((nvar, n, n), np.complex64)] * 2,
+                       propagators=[J.CalculatePropagator(0.19525, 0.0, 0.0, 1)], proidx=[0, 0],
+                       objslc=np.array([0, 1], np.int32), det_objslc=configs.detection_planes(0, 2)[0], ndetslc=1,
+                       detectors=[J.CalculateRingDetector(70.0, 200.0)],
+                       probe=J.CalculateProbeWaveFourier(25.0, 0.03), slice_thickness=0.19525)
+    eng = Engine().setup(sp)
+    qx = E.freqs(n, a).astype(np.float64)
+    q2 = qx[None, :] ** 2 + qx[:, None] ** 2
+    # Sr, Ti, O: Gaussian form factors scaled with Z^0.7, 6% absorptive part
+    ff = np.stack([1.2 * z ** 0.7 * np.exp(-2 * np.pi ** 2 * 0.02 ** 2 * q2) * (1 + 0.06j) for z in (38, 22, 8)])
+    ff = ff.astype(np.complex64)
+    cell = {0: [(0.0, 0.0, 0), (0.5, 0.5, 2)],                  # SrO: Sr, O
+            1: [(0.5, 0.0, 2), (0.0, 0.5, 2), (0.5, 0.5, 1)]}   # TiO2: O, O, Ti (shifted by half a cell)
+    rng = np.random.default_rng(21)
+    for t in (0, 1):
+        ix, iy = np.meshgrid(np.arange(8), np.arange(8), indexing="xy")
+        base = np.concatenate([np.stack([(ix.ravel() + fx) * 0.3905, (iy.ravel() + fy) * 0.3905], 1)
+                               for fx, fy, _ in cell[t]])
+        typ = np.concatenate([np.full(64, k, np.int32) for _, _, k in cell[t]])
+        sets = [((base + rng.normal(0.0, 0.008, base.shape)).astype(np.float32), typ, None) for _ in range(nvar)]
+        eng.formfactor_gratings(t, ff, sets, 0.19525)
+        dev = eng.gratings(t, nvar)
+        for v, (xy, ty, _) in enumerate(sets):
+            ref = E.phase_grating(n, n, a, a, 0.19525, wl, ff, xy, ty)
+            e = rel_l2(dev[v], ref)
+            ph = np.abs(np.angle(ref)).max()
+            if v in (0, nvar - 1):
+                print(f"type {t} variant {v}: rel L2 {e:.2e}, max |phase| {ph:.2f} rad")
+            assert ph > 0.3
+            assert e < 1e-4
+    eng.close()
