@@ -340,6 +340,12 @@ class Engine:
         ff = np.ascontiguousarray(ff2d, np.complex64)
         if ff.ndim != 3 or ff.shape[1:] != (self.sp.ny, self.sp.nx):
             raise ValueError(f"ff2d must be (ntype, {self.sp.ny}, {self.sp.nx})")
+        if not sites:
+            raise ValueError("no atom sets")
+        for x, t, o in sites:
+            k = len(np.asarray(t).reshape(-1))
+            if np.asarray(x).size != 2 * k or (o is not None and np.asarray(o).size != k):
+                raise ValueError("each atom set needs 2 coordinates and one occupancy per type entry")
         nat = _i32([len(t) for _, t, _ in sites])
         xy = _f32(np.concatenate([np.asarray(x, np.float32).reshape(-1, 2) for x, _, _ in sites]).reshape(-1))
         typ = _i32(np.concatenate([np.asarray(t).reshape(-1) for _, t, _ in sites]))

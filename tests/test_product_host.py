@@ -162,3 +162,23 @@ def test_aberrated_probe_bit_exact(seed):
     r = O.probe_wave(64, 48, 1.7, 1.3, M.GetWaveLength(), 22.0, 0.03, 0.05, 15.0, 0.0, 0.0, 0.5, -0.5, abrr=ab)
     assert np.array_equal(p.view(np.uint32), r.view(np.uint32))
     assert abs(np.sum(np.abs(p.astype(np.complex128)) ** 2) - 1.0) < 1e-5
+
+
+def test_formfactor_gratings_rejects_inconsistent_atom_sets():
+    """Engine.formfactor_gratings checks the per-set array lengths before any
+    device call (the C-ABI takes plain pointers and cannot)"""
+    from jmultislicelib_amd.jms import Engine
+
+    e = Engine.__new__(Engine)  # no device set-up: validation comes first
+
+    class _Grid:
+        nx = ny = 8
+
+    e.sp = _Grid()
+    ff = np.zeros((1, 8, 8), np.complex64)
+    for bad in ([], [(np.zeros((3, 2)), np.zeros(2, int), None)],
+                [(np.zeros((2, 2)), np.zeros(2, int), np.ones(3))]):
+        with pytest.raises(ValueError):
+            e.formfactor_gratings(0, ff, bad, 0.2)
+    with pytest.raises(ValueError):
+        e.formfactor_gratings(0, np.zeros((1, 8, 4), np.complex64), [(np.zeros((1, 2)), [0], None)], 0.2)
