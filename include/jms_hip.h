@@ -36,7 +36,8 @@ typedef struct jms_hip_engine jms_hip_engine;
 /* engine lifetime ---------------------------------------------------------- */
 int jms_hip_create(jms_hip_engine** out);
 void jms_hip_destroy(jms_hip_engine* e);
-/* 1 when nx (row length) and ny (column length) have compiled kernels */
+/* 1 when nx (row length) and ny (column length) have compiled kernels:
+ * 16 32 48 64 96 128 160 192 256 320 360 384 480 512 625 640 720 768 1024 2048 */
 int jms_hip_grid_supported(int nx, int ny);
 
 /* setup (replaces the device side of the reference setup chain,

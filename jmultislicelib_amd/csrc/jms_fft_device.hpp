@@ -341,6 +341,7 @@ template <> struct PlanFor<500> { using type = Radices<10, 10, 5>; };
 template <> struct PlanFor<512> { using type = Radices<8, 8, 8>; };
 template <> struct PlanFor<576> { using type = Radices<8, 8, 9>; };
 template <> struct PlanFor<600> { using type = Radices<8, 15, 5>; };
+template <> struct PlanFor<625> { using type = Radices<5, 5, 5, 5>; };
 template <> struct PlanFor<640> { using type = Radices<8, 8, 10>; };
 template <> struct PlanFor<720> { using type = Radices<8, 9, 10>; };
 template <> struct PlanFor<768> { using type = Radices<8, 12, 8>; };

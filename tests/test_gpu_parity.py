@@ -21,7 +21,8 @@ TOL = 1e-4
 
 
 @pytest.mark.parametrize("ny,nx", [(64, 64), (256, 256), (512, 512), (360, 384), (128, 256),
-                                   (720, 640), (96, 480), (2048, 64), (16, 768), (320, 32), (48, 128)])
+                                   (720, 640), (96, 480), (2048, 64), (16, 768), (320, 32), (48, 128),
+                                   (625, 625), (48, 625)])
 def test_fft2d_vs_oracle(ny, nx):
     rng = np.random.default_rng(nx * 7 + ny)
     a = (rng.standard_normal((2, ny, nx)) + 1j * rng.standard_normal((2, ny, nx))).astype(np.complex64)
@@ -140,7 +141,7 @@ def test_staged_scan_vs_oracle(graph):
 @pytest.mark.parametrize("nx,ny", [(128, 64), (64, 128), (360, 384), (96, 160), (2048, 64), (64, 2048),
                                    (1024, 96), (96, 1024), (768, 640), (360, 1024), (48, 2048),
                                    (384, 360), (96, 480), (480, 720), (720, 64), (320, 48), (48, 320),
-                                   (32, 128)])
+                                   (32, 128), (625, 625), (625, 96), (64, 625)])
 def test_nonsquare_mixed_radix(nx, ny):
     sp = random_setup(nx, ny, nslices=6, nvar=(3, 2), ndetper=2, seed=nx + ny)
     var = configs.variant_sequences(sp, 2, seed=3)
@@ -193,5 +194,37 @@ def test_full_2048_grid_batch_vs_oracle():
     ref = O.stem_scan(sp, dx, dy, np.ones(1, np.float32), var, prec=32, nthreads=2)
     e = rel_l2(g, ref)
     print(f"2048^2 batch: rel L2 {e:.2e}")
+    assert e < TOL
+    eng.close()
+
+
+def test_reference_sample_job_shape_625():
+    """The shape of the reference's own sample job (test/JMSBench1/test with
+    STO_test.slp): 625^2 pixels over 4x4 SrTiO3 cells at 300 kV, 64 slices
+    with a detection plane every 4, 25 mrad, BF 0-10 / ABF 12-24 / HAADF
+    80-200 mrad, positions on the 40x40 raster over 0.782 nm.  The slice
+    files themselves are not part of the reference, so the gratings come
+    from the Gaussian potential model, with 3 frozen-phonon variants per
+    slice type (the listing has 150); a staged scan of 5 positions x 2
+    passes against the oracle."""
+    sp = configs.perovskite_setup(n=625, ncell=4, nslices=64, nvar=3, ht=300.0, alpha=25.0, ndetper=4,
+                                  det_ranges=((0.0, 10.0), (12.0, 24.0), (80.0, 200.0)))
+    assert sp.ndetslc == 17
+    step = 0.782 / 40
+    idx = np.array([0, 7, 19, 26, 39])
+    dx = (idx * step).astype(np.float32)
+    dy = (idx[::-1] * step).astype(np.float32)
+    npass = 2
+    weights = np.full(npass, 1.0 / npass, np.float32)
+    var = configs.variant_sequences(sp, len(dx) * npass, seed=7)
+    eng = Engine().setup(sp)
+    eng.set_batch(4)  # three batches, the last one partial
+    eng.stage_scan(dx, dy, None, weights, var)
+    eng.run_staged(0, len(dx))
+    eng.sync()
+    g = eng.scan_results()
+    ref = O.stem_scan(sp, dx, dy, weights, var, prec=32, nthreads=8)
+    e = rel_l2(g, ref)
+    print(f"625^2 sample-job shape: {g.shape} results, rel L2 {e:.2e}")
     assert e < TOL
     eng.close()
