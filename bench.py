@@ -225,7 +225,7 @@ def run_surrogate(args):
     smp = S.simulate(train, args.train_positions, seed=5, eng=eng)
     t1 = time.perf_counter()
     log(f"training data: {len(smp.t)} positions on {len(train)} lattices in {t1 - t0:.1f} s")
-    ens = S.Ensemble(nmembers=4, max_batch=256, seed=1)
+    ens = S.Ensemble(nmembers=4, max_batch=256, seed=1, step=args.window_step)
     hist = ens.train(smp, epochs=args.epochs, batch=256, lr=1e-3, log=log)
     t2 = time.perf_counter()
     log(f"training: {args.epochs} epochs x 4 members in {t2 - t1:.1f} s; final losses {hist[-1]}")
@@ -240,7 +240,7 @@ def run_surrogate(args):
     t3 = time.perf_counter()
     plain = S.plain_scan(eng, px, py)
     t4 = time.perf_counter()
-    ws = S.Samples([test_lat], np.zeros(len(px), np.int32), px, py, plain).windows()
+    ws = S.Samples([test_lat], np.zeros(len(px), np.int32), px, py, plain).windows(step=ens.step)
     net_only, spread = ens.predict(ws)
     ws.close()
     eng.set_batch(2048)
@@ -261,6 +261,8 @@ def run_surrogate(args):
                                        f"positions, unseen lattice {ns}x{ns} raster over one cell",
                            "positions": len(px), "plain_s": round(t_plain, 3), "surrogate_s": round(t_surr, 3),
                            "resimulated": int(len(idx)), "audit": 0.05, "spread_max": 0.02,
+                           "window": {"pixels": [S.WIN, S.WIN], "step_ang": ens.step,
+                                      "configured": abs(ens.step - 0.1) < 1e-9},
                            "errors": errs, "train_data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
                            "timings": {k: round(v, 3) for k, v in tm.items()},
                            "test_lattice": {"a": test_lat.a, "za": test_lat.za, "zb": test_lat.zb}},
@@ -287,6 +289,8 @@ def main():
     ap.add_argument("--train-positions", type=int, default=400, help="surrogate: positions per lattice")
     ap.add_argument("--epochs", type=int, default=30, help="surrogate: training epochs")
     ap.add_argument("--scan", type=int, default=256, help="surrogate: evaluation scan is scan x scan")
+    ap.add_argument("--window-step", type=float, default=0.1,
+                    help="surrogate: window pixel spacing [A] (0.1 A: 32 x 32 pixels over 3.2 A, the configured input)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling HBM counter readings) and exit")

@@ -124,9 +124,10 @@ class WindowSet:
         return cls(h, x.shape[0])
 
     @classmethod
-    def from_crystals(cls, maps, cells, cid, px_ang, py_ang, binw, t=None):
+    def from_crystals(cls, maps, cells, cid, px_ang, py_ang, binw, t=None, step=STEP_ANG):
         """maps [ncrys][2][m][m] (potential_maps), cells [ncrys][2] [A], per
-        sample crystal index cid and probe position [A]"""
+        sample crystal index cid and probe position [A]; window pixel
+        spacing step [A]"""
         maps = np.ascontiguousarray(maps, np.float32)
         cells = np.ascontiguousarray(cells, np.float32)
         cid = np.ascontiguousarray(cid, np.int32)
@@ -136,7 +137,7 @@ class WindowSet:
         h = C.c_void_p()
         _chk("jms_nn_data_windows", lib().jms_nn_data_windows(
             C.byref(h), maps.shape[0], maps.shape[1], maps.shape[2], ptr(maps), ptr(cells), len(px), ptr(cid),
-            ptr(px), ptr(py), ptr(bw), STEP_ANG, WIN_SCALE, ptr(tt)))
+            ptr(px), ptr(py), ptr(bw), float(step), WIN_SCALE, ptr(tt)))
         return cls(h, len(px))
 
     def windows(self) -> np.ndarray:
@@ -258,13 +259,13 @@ class Samples:
     py: np.ndarray
     t: np.ndarray       # [n][3]
 
-    def windows(self, nslices: int = 200, scale=None) -> WindowSet:
-        """device windows with targets t, or log(t / scale) per detector when
-        scale is given"""
+    def windows(self, nslices: int = 200, scale=None, step: float = STEP_ANG) -> WindowSet:
+        """device windows (pixel spacing step [A]) with targets t, or
+        log(t / scale) per detector when scale is given"""
         maps = np.stack([potential_maps(lat) for lat in self.lattices])
         cells = np.array([[lat.a, lat.a] for lat in self.lattices], np.float32)
         t = self.t if scale is None else np.log(np.maximum(self.t, 1e-30) / np.asarray(scale, np.float32))
-        return WindowSet.from_crystals(maps, cells, self.cid, self.px, self.py, bin_weights(nslices), t)
+        return WindowSet.from_crystals(maps, cells, self.cid, self.px, self.py, bin_weights(nslices), t, step)
 
 
 def simulate(lattices, npos: int, seed: int, eng: Engine = None, **kw) -> Samples:
@@ -289,11 +290,13 @@ def simulate(lattices, npos: int, seed: int, eng: Engine = None, **kw) -> Sample
 
 
 class Ensemble:
-    """Networks trained on the same data from different initial weights."""
+    """Networks trained on the same data from different initial weights, on
+    windows of pixel spacing step [A] (training and scans use the same)."""
 
-    def __init__(self, nmembers: int = 4, max_batch: int = 256, seed: int = 1):
+    def __init__(self, nmembers: int = 4, max_batch: int = 256, seed: int = 1, step: float = STEP_ANG):
         self.nets = [Network(max_batch, seed + i) for i in range(nmembers)]
         self.scale = np.ones(NOUT, np.float32)
+        self.step = float(step)
 
     def train(self, samples: Samples, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0,
               nslices: int = 200, log=None):
@@ -302,7 +305,7 @@ class Ensemble:
         measures relative errors; the learning rate decays linearly to lr/10.
         Returns the per-epoch losses [epochs][members]."""
         self.scale = np.asarray(samples.t, np.float64).mean(axis=0).astype(np.float32)
-        data = samples.windows(nslices, self.scale)
+        data = samples.windows(nslices, self.scale, self.step)
         one = np.ones(NOUT, np.float32)
         hist = []
         for ep in range(epochs):
@@ -337,7 +340,7 @@ def surrogate_scan(ens: Ensemble, eng: Engine, lat: Lattice, px_ang, py_ang, aud
     t0 = time.perf_counter()
     n = len(px_ang)
     ws = Samples([lat], np.zeros(n, np.int32), np.asarray(px_ang, np.float32), np.asarray(py_ang, np.float32),
-                 np.zeros((n, NOUT), np.float32)).windows(nslices)
+                 np.zeros((n, NOUT), np.float32)).windows(nslices, step=ens.step)
     t1 = time.perf_counter()
     mean, spread = ens.predict(ws)
     ws.close()
