@@ -105,6 +105,12 @@ template <> struct Tuned<512> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
 };
+template <> struct Tuned<625> {
+  static constexpr int LPB = 4;
+  static constexpr bool GTW = false;
+  static constexpr int CW = 16;
+  static constexpr int WPE = 1;
+};
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
   static constexpr bool GTW = true;
