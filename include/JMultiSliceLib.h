@@ -13,6 +13,10 @@
  *  - OffsetIncomingWave takes bcppl and SetIncidentWave takes bTranspose (the
  *    reference wrappers drop them, JMultiSliceLib.h:246,326).
  *  - whichcode _JMS_CODE_CPU (1) is rejected: this library has no CPU path.
+ *    The CPU-only entries (SetIncomingWaveCPU, CPUMultislice, ClearDetMem_h)
+ *    are exported and return 1.
+ *  - SetPlasmonMC is exported; activating it makes GPUMultislice fail
+ *    (plasmon Monte-Carlo is not part of this library).
  *  - GPUMultisliceScan / GPUMultisliceScanVar are additive batched entries.
  */
 #ifndef JMULTISLICELIB_HIP_H
@@ -28,8 +32,9 @@ extern "C" {
 
 /* Probe parameters (fields of CJProbeParams, JProbeGen.h:84-122).  Angles in
  * mrad except m_alpha_adir [deg]; m_wl is set by the library.  Aberrations:
- * m_abrr_num coefficient pairs (x, y) in m_abrr_coeff; only all-zero sets are
- * supported. */
+ * m_abrr_num coefficient pairs (x, y) [nm] in m_abrr_coeff, in the order of
+ * the reference's aberration table (JProbeGen.cpp:508-571: order m = 1..8,
+ * n <= m, m+n even; up to 24 terms). */
 typedef struct CJProbeParams {
   float m_wl;
   float m_alpha;
@@ -71,6 +76,8 @@ JMS_API void SetSupercellSize(float* a0);                           /* :138 */
 JMS_API void SetSupercellSizeABC(float a, float b, float c);        /* :141 */
 JMS_API void SetSliceThickness(int islc, float fthickness);         /* :146 */
 JMS_API void DiffractionDescan(int bActivate);                      /* :149 */
+JMS_API void SetPlasmonMC(int bActivate, float fq_e, float fq_c, float fmfp,
+                          unsigned int nexmax);                       /* :152 */
 JMS_API void SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany,
                                    int iThread);                      /* :160 */
 JMS_API void SetDiffractionDescan(int whichcode, float descanx, float descany,
@@ -115,6 +122,7 @@ JMS_API int ResetWaveAveraging(int whichcode, int iThread);         /* :272 */
 /* calculation interface */
 JMS_API int CalculateProbeWaveFourier(CJProbeParams* prm, float* wav);   /* :280 */
 JMS_API int SetIncomingWaveGPU(float* wav, int bTranspose);              /* :286 */
+JMS_API int SetIncomingWaveCPU(float* wav, int bTranspose, int iThread); /* :293, returns 1 */
 JMS_API int CalculatePropagator(float fthick, float otx, float oty, float* pro,
                                 int ntype);                              /* :303 */
 JMS_API int CalculateRingDetector(float beta0, float beta1, float phi0, float phi1,
@@ -122,10 +130,12 @@ JMS_API int CalculateRingDetector(float beta0, float beta1, float phi0, float ph
                                   float* det, int* msklen, int* msk, int kmom1); /* :318 */
 JMS_API int OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int iThread,
                                int ibtx, int ibty, int bcppl);           /* :326 */
+JMS_API int CPUMultislice(int islc0, int accmode, float weight, int iThread); /* :333, returns 1 */
 JMS_API int GPUMultislice(int islc0, int accmode, float weight);         /* :339 */
 JMS_API int GetResult(int whichcode, int whichresult, float* dst, int iThread); /* :346 */
 JMS_API int GetAvgResult(int whichcode, int whichresult, float* dst, float* wgt,
                          int iThread);                                  /* :354 */
+JMS_API int ClearDetMem_h(int iThread);                                  /* :357, returns 1 */
 JMS_API int ClearDetMem_d(void);                                         /* :363 */
 
 /* batched STEM scan (no counterpart in the reference; additive).  For each

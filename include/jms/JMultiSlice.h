@@ -6,15 +6,17 @@
 // include/jms_hip.h (HIP kernels for gfx950).
 //
 // Differences from the reference, by design:
-//  - The CPU code path (_JMS_CODE_CPU: CPUMultislice and the host multislice
-//    buffers) is not part of this library; requests for it return an error.
+//  - The CPU code path (_JMS_CODE_CPU: CPUMultislice, SetIncomingWaveCPU,
+//    ClearDetMem_h and the host multislice buffers) is not part of this
+//    library; its entry points exist and return an error (1).
 //  - Plasmon Monte-Carlo and sub-frame phase gratings are not implemented;
 //    setups that need them fail with an error.
 //  - The detection channels (integrated, image, diffraction, real- and
 //    Fourier-space waves and their averages) are filled by the single-wave
 //    path (GPUMultislice); the batched scan fills the integrated channel.
-//  - Probe aberrations other than none are not implemented
-//    (CalculateProbeWaveFourier returns an error if a coefficient is set).
+//  - CalculateProbeWaveFourier takes the axial aberrations of the
+//    reference's probe generator (24 terms, m = 1..8, n <= m, m+n even);
+//    partial spatial and temporal coherence envelopes are not applied.
 //  - Additive batched entry GPUMultisliceScan runs many positions and passes
 //    in one batched schedule; a batch of one gives the numbers of
 //    OffsetIncomingWave + GPUMultislice.
@@ -130,11 +132,15 @@ class CJMultiSlice {
 
   // detector memory
   int ClearDetMem_d(void);
+  int ClearDetMem_h(int iThread);  // CPU path: returns 1 (JMultiSlice.cpp:3128)
 
   // multislice
   int OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int iThread = 0, int ibtx = 0,
                          int ibty = 0, bool bcppl = true);
   int SetIncomingWaveGPU(fcmplx* wav, bool bTranspose = false);
+  // CPU path: returns 1 (JMultiSlice.cpp:3541, :3589)
+  int SetIncomingWaveCPU(fcmplx* wav, bool bTranspose = false, int iThread = 0);
+  int CPUMultislice(int islc0, int accmode, float weight = 1.0f, int iThread = 0);
   int GPUMultislice(int islc0, int accmode, float weight = 1.0f);
   int GetResult(int whichcode, int whichresult, float* dst, int iThread = 0);
   // averaging channels (JMultiSlice.h:903-925)

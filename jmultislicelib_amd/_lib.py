@@ -39,6 +39,7 @@ _SIGS = [
     ("SetSupercellSizeABC", None, [c_float, c_float, c_float]),
     ("SetSliceThickness", None, [c_int, c_float]),
     ("DiffractionDescan", None, [c_int]),
+    ("SetPlasmonMC", None, [c_int, c_float, c_float, c_float, C.c_uint]),
     ("SetDiffractionDescanN", None, [c_int, c_int, c_int, c_int]),
     ("SetDiffractionDescan", None, [c_int, c_float, c_float, c_int]),
     ("SetDiffractionDescanMRad", None, [c_int, c_float, c_float, c_int]),
@@ -71,13 +72,16 @@ _SIGS = [
     ("ResetWaveAveraging", c_int, [c_int, c_int]),
     ("CalculateProbeWaveFourier", c_int, [c_void_p, c_void_p]),
     ("SetIncomingWaveGPU", c_int, [c_void_p, c_int]),
+    ("SetIncomingWaveCPU", c_int, [c_void_p, c_int, c_int]),
     ("CalculatePropagator", c_int, [c_float, c_float, c_float, c_void_p, c_int]),
     ("CalculateRingDetector", c_int, [c_float, c_float, c_float, c_float, c_float, c_float, c_char_p,
                                       c_void_p, c_void_p, c_void_p, c_int]),
     ("OffsetIncomingWave", c_int, [c_int, c_float, c_float, c_float, c_int, c_int, c_int, c_int]),
+    ("CPUMultislice", c_int, [c_int, c_int, c_float, c_int]),
     ("GPUMultislice", c_int, [c_int, c_int, c_float]),
     ("GetResult", c_int, [c_int, c_int, c_void_p, c_int]),
     ("GetAvgResult", c_int, [c_int, c_int, c_void_p, c_void_p, c_int]),
+    ("ClearDetMem_h", c_int, [c_int]),
     ("ClearDetMem_d", c_int, []),
     ("GPUMultisliceScan", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("GPUMultisliceScanVar", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,

@@ -802,6 +802,10 @@ int CJMultiSlice::ClearDetMem_d(void) {
   return 0;
 }
 
+int CJMultiSlice::ClearDetMem_h(int) { return reject_cpu("ClearDetMem_h"); }
+int CJMultiSlice::SetIncomingWaveCPU(fcmplx*, bool, int) { return reject_cpu("SetIncomingWaveCPU"); }
+int CJMultiSlice::CPUMultislice(int, int, float, int) { return reject_cpu("CPUMultislice"); }
+
 int CJMultiSlice::OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int iThread, int ibtx,
                                      int ibty, bool bcppl) {
   (void)iThread;

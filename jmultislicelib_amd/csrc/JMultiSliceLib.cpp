@@ -28,6 +28,9 @@ void SetSupercellSize(float* a0) { jms_obj().SetSupercellSize(a0); }
 void SetSupercellSizeABC(float a, float b, float c) { jms_obj().SetSupercellSize(a, b, c); }
 void SetSliceThickness(int islc, float t) { jms_obj().SetSliceThickness(islc, t); }
 void DiffractionDescan(int bActivate) { jms_obj().DiffractionDescan(bActivate != 0); }
+void SetPlasmonMC(int bActivate, float fq_e, float fq_c, float fmfp, unsigned int nexmax) {
+  jms_obj().SetPlasmonMC(bActivate != 0, fq_e, fq_c, fmfp, nexmax);
+}
 void SetDiffractionDescanN(int whichcode, int ndescanx, int ndescany, int iThread) {
   jms_obj().SetDiffractionDescanN(whichcode, ndescanx, ndescany, iThread);
 }
@@ -103,6 +106,9 @@ int ResetWaveAveraging(int whichcode, int iThread) { return jms_obj().ResetWaveA
 int CalculateProbeWaveFourier(CJProbeParams* prm, float* wav) {
   return jms_obj().CalculateProbeWaveFourier(prm, (fcmplx*)wav);
 }
+int SetIncomingWaveCPU(float* wav, int bTranspose, int iThread) {
+  return jms_obj().SetIncomingWaveCPU((fcmplx*)wav, bTranspose != 0, iThread);
+}
 int SetIncomingWaveGPU(float* wav, int bTranspose) {
   return jms_obj().SetIncomingWaveGPU((fcmplx*)wav, bTranspose != 0);
 }
@@ -120,6 +126,9 @@ int CalculateRingDetector(float beta0, float beta1, float phi0, float phi1, floa
 int OffsetIncomingWave(int whichcode, float dx, float dy, float dz, int iThread, int ibtx, int ibty, int bcppl) {
   return jms_obj().OffsetIncomingWave(whichcode, dx, dy, dz, iThread, ibtx, ibty, bcppl != 0);
 }
+int CPUMultislice(int islc0, int accmode, float weight, int iThread) {
+  return jms_obj().CPUMultislice(islc0, accmode, weight, iThread);
+}
 int GPUMultislice(int islc0, int accmode, float weight) { return jms_obj().GPUMultislice(islc0, accmode, weight); }
 int GetAvgResult(int whichcode, int whichresult, float* dst, float* wgt, int iThread) {
   float w = 0.f;
@@ -131,6 +140,7 @@ int GetAvgResult(int whichcode, int whichresult, float* dst, float* wgt, int iTh
 int GetResult(int whichcode, int whichresult, float* dst, int iThread) {
   return jms_obj().GetResult(whichcode, whichresult, dst, iThread);
 }
+int ClearDetMem_h(int iThread) { return jms_obj().ClearDetMem_h(iThread); }
 int ClearDetMem_d(void) { return jms_obj().ClearDetMem_d(); }
 
 int GPUMultisliceScan(int npos, const float* dx, const float* dy, const float* dz, int npass,

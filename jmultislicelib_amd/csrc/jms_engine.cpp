@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -95,9 +96,13 @@ struct jms_hip_engine {
   float w_imgavg = 0.f, w_wavavg = 0.f;
   // diffraction de-scan [pixel] of the Fourier-space readouts
   int dsx = 0, dsy = 0;
-  // graph cache: key = (nwave, islc0)
+  // graph cache, key = (nwave, islc0, positions, passes).  A captured launch
+  // sequence holds raw device addresses, so the cache is also tied to the
+  // addresses of every buffer the batch reads or writes (graph_bufs): when
+  // any of them moves, all graphs are dropped before the next capture.
   bool use_graph = false;
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  std::map<std::array<int, 4>, hipGraphExec_t> graphs;
+  std::array<const void*, 16> graph_bufs{};
   // timing
   bool timing = false;
   Timer tm[3];
@@ -120,6 +125,11 @@ struct jms_hip_engine {
   void drop_graphs() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
+  }
+  // device addresses a captured batch depends on
+  std::array<const void*, 16> batch_buffers() const {
+    return {waves.p, partial.p, cur_ppar.p, cur_var.p, cur_res.p, s_weights.p, wav0.p, pgr.p,
+            pro.p, pro_t.p, detw.p, detw_t.p, twx.p, twy.p, cppl.p, cppl_t.p};
   }
   int ntiles_col() const { return jms::col_tiles(ny, nx); }
 };
@@ -861,7 +871,12 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
       if (err == hipSuccess && S > 0)
         err = hipMemcpyAsync(e->cur_var.p, var, sizeof(int) * (size_t)nw * S, hipMemcpyDeviceToDevice, e->st);
       if (err != hipSuccess) return e->fail(120, "stage batch", err);
-      auto key = std::make_pair(nw, islc0);
+      const auto bufs = e->batch_buffers();
+      if (bufs != e->graph_bufs) {
+        e->drop_graphs();
+        e->graph_bufs = bufs;
+      }
+      const std::array<int, 4> key{nw, islc0, np, npass};
       auto it = e->graphs.find(key);
       if (it == e->graphs.end()) {
         hipGraph_t g = nullptr;

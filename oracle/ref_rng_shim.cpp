@@ -14,4 +14,9 @@ double ref_rng_unirand(void* r, double a, double b) { return static_cast<CRng*>(
 double ref_rng_unirand_oub(void* r, double a, double b) { return static_cast<CRng*>(r)->unirand_oub(a, b); }
 double ref_rng_normrand(void* r, double mu, double s) { return static_cast<CRng*>(r)->normrand(mu, s); }
 int ref_rng_poisrand(void* r, double mu) { return static_cast<CRng*>(r)->poisrand(mu); }
+// n consecutive draws of unirand(a, b) (oub != 0: unirand_oub) into out
+void ref_rng_unirand_n(void* r, double a, double b, int n, int oub, double* out) {
+  CRng* g = static_cast<CRng*>(r);
+  for (int i = 0; i < n; ++i) out[i] = oub ? g->unirand_oub(a, b) : g->unirand(a, b);
+}
 }

@@ -12,6 +12,36 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
+def det_errors(g, r) -> np.ndarray:
+    """Relative L2 error of every (detection plane, detector) pair: the last
+    axis of a readout array indexes detectors, the one before it (if any)
+    detection planes, and any axes in front (probe positions) are summed
+    over.  A pair whose oracle values are all zero gets the largest absolute
+    GPU value instead (it must be exactly zero too)."""
+    g = np.asarray(g, np.float64)
+    r = np.asarray(r, np.float64)
+    assert g.shape == r.shape, (g.shape, r.shape)
+    if g.ndim == 1:
+        g, r = g[None, None], r[None, None]
+    elif g.ndim == 2:
+        g, r = g[None], r[None]
+    g = g.reshape(-1, g.shape[-2], g.shape[-1])
+    r = r.reshape(-1, r.shape[-2], r.shape[-1])
+    num = np.sqrt(((g - r) ** 2).sum(0))
+    den = np.sqrt((r ** 2).sum(0))
+    zero = den == 0
+    return np.where(zero, np.abs(g).max(0), num / np.where(zero, 1.0, den))
+
+
+def assert_dets(g, r, tol, what=""):
+    """Every detector at every detection plane within `tol` relative L2 of
+    the oracle (see det_errors); returns the error table."""
+    e = det_errors(g, r)
+    print(f"{what} per-detector rel. L2 (plane x detector): {np.array2string(e, precision=2)}")
+    assert np.all(e < tol), f"{what}: detector error {e.max():.3e} >= {tol}"
+    return e
+
+
 def random_setup(nx, ny, nslices=8, nvar=(3, 2), ndetper=0, seed=0, flags=1, ht=200.0,
                  ptype=1, det_ranges=((0, 10), (10, 25), (30, 120))) -> Setup:
     """Generic setup on an nx x ny grid: random weak phase gratings, library

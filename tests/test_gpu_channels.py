@@ -9,14 +9,16 @@
     ReadoutDetAvg :2667): accumulated over runs until reset, with the summed
     run weights.
 Tolerance: relative L2 <= 1e-4 against the oracle's float32 run; the FP64
-twin's error is printed beside it."""
+twin's error is printed beside it and asserted: GPU and float32 oracle both
+within 1e-4 of it, the GPU's error at most 4x the oracle's or 2e-6.
+Integrated detectors are checked one by one (and plane by plane)."""
 import numpy as np
 import pytest
 
 from jmultislicelib_amd import Engine, JMultiSlice, configs
 from oracle import oracle as O
 
-from helpers import random_setup, rel_l2
+from helpers import assert_dets, random_setup, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -39,15 +41,20 @@ def test_all_channels_single_wave(nx, ny, ndetper):
         win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.07, -0.05)
         ref32 = O.multislice_channels(sp, win, var[r], w, prec=32, flags=ALL, chans=ref32)
         ref64 = O.multislice_channels(sp, win, var[r], w, prec=64, flags=ALL, chans=ref64)
-    e_int = rel_l2(eng.result(1), ref32["int"][:, :len(sp.detectors)])
-    print(f"{nx}x{ny} ndetper {ndetper}: integrated {e_int:.2e}")
-    assert e_int < TOL
+    nd = len(sp.detectors)
+    g_int = eng.result(1)
+    assert_dets(g_int, ref32["int"][:, :nd], TOL, f"{nx}x{ny} ndetper {ndetper} integrated")
+    e_g64 = assert_dets(g_int, ref64["int"][:, :nd], TOL, "integrated gpu vs FP64")
+    e_o64 = assert_dets(ref32["int"][:, :nd], ref64["int"][:, :nd], TOL, "integrated oracle f32 vs FP64")
+    assert np.all(e_g64 <= np.maximum(4.0 * e_o64, 2e-6))
     for k, flag in KEYS.items():
         g = eng.result(flag)
         e = rel_l2(g, ref32[k])
         e64 = rel_l2(g, ref64[k])
-        print(f"  {k}: rel L2 vs oracle f32 {e:.2e}, vs FP64 twin {e64:.2e}")
-        assert e < TOL, k
+        o64 = rel_l2(ref32[k], ref64[k])
+        print(f"  {k}: rel L2 vs oracle f32 {e:.2e}, vs FP64 twin: gpu {e64:.2e}, oracle f32 {o64:.2e}")
+        assert e < TOL and e64 < TOL and o64 < TOL, k
+        assert e64 <= max(4.0 * o64, 2e-6), k
     eng.close()
 
 
@@ -74,7 +81,8 @@ def test_averaging_channels():
     det = np.stack([d for d, _ in sp.detectors]).astype(np.float64)
     el_int = np.einsum("pyx,dyx->pd", np.abs(wff) ** 2, det)
     got, w = eng.readout_det_avg(1)
-    assert rel_l2(got, el_int) < TOL and abs(w - 1.0) < 1e-6
+    assert_dets(got, el_int, TOL, "elastic (averaged-wave) detector sums")
+    assert abs(w - 1.0) < 1e-6
     got, _ = eng.readout_det_avg(4)
     assert rel_l2(got, np.abs(wff) ** 2) < TOL
     got, _ = eng.readout_det_avg(2)

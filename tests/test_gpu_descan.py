@@ -10,7 +10,7 @@ import pytest
 from jmultislicelib_amd import Engine, JMultiSlice, configs
 from oracle import oracle as O
 
-from helpers import random_setup, rel_l2
+from helpers import assert_dets, random_setup, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -27,9 +27,7 @@ def test_descan_engine(nx, ny, d):
     eng.multislice(var, weight=0.8)
     win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.05, 0.08)
     ref = O.multislice_channels(sp, win, var, 0.8, prec=32, flags=ALL, descan=d)
-    e_int = rel_l2(eng.result(1), ref["int"][:, :len(sp.detectors)])
-    print(f"{nx}x{ny} descan {d}: integrated {e_int:.2e}")
-    assert e_int < 1e-4
+    assert_dets(eng.result(1), ref["int"][:, :len(sp.detectors)], 1e-4, f"{nx}x{ny} descan {d}")
     for k, flag in (("dif", 4), ("wff", 16), ("img", 2), ("wfr", 8)):
         e = rel_l2(eng.result(flag), ref[k])
         print(f"  {k}: {e:.2e}")
@@ -70,7 +68,7 @@ def test_descan_class_api():
     got = M.GetResult(2, 1)
     win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.1, 0.1)
     ref = O.multislice_channels(sp, win, var, 1.0, prec=32, flags=1, descan=(4, -3))
-    assert rel_l2(got, ref["int"][:, :3]) < 1e-4
+    assert_dets(got, ref["int"][:, :3], 1e-4, "class API descan")
     # the flag is a setting of the process-global object (Cleanup keeps it, as
     # in the reference): switch it off for later users
     M.DiffractionDescan(False)

@@ -7,7 +7,7 @@ import pytest
 from jmultislicelib_amd import Engine, configs, io as jio
 from oracle import oracle as O
 
-from helpers import rel_l2
+from helpers import assert_dets
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,4 @@ def test_slice_file_specimen_vs_oracle(tmp_path):
     for r in range(2):
         win = O.offset_wave(s2.probe, s2.a0[0], s2.a0[1], s2.wl, 0.2, 0.1)
         ref += O.multislice(s2, win, var[r], 0.5, prec=32)[0]
-    e = rel_l2(res[1], ref)
-    print(f"slice-file specimen: rel L2 {e:.2e}")
-    assert e < 1e-4
+    assert_dets(res[1], ref, 1e-4, "slice-file specimen")

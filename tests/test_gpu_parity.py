@@ -2,9 +2,13 @@
 
 Tolerances (float32 arithmetic on both sides):
   * 2-D FFT: relative L2 error <= 2e-6 against the oracle's FP64 transform;
-  * multislice: detector intensities and exit waves within 1e-4 relative L2
-    of the oracle's float32 run of the same recipe (CPUMultislice restated),
-    variant sequences identical (drawn on the host from the same seed).
+  * multislice: every detector at every detection plane within 1e-4
+    relative L2 of the oracle's float32 run of the same recipe
+    (CPUMultislice restated; a detector whose oracle values are all zero
+    must be zero), exit waves within 1e-4 relative L2, variant sequences
+    identical (drawn on the host from the same seed);
+  * against the oracle's FP64 twin: GPU and float32 oracle both within 1e-4,
+    the GPU's error at most 4x the float32 oracle's or 2e-6.
 """
 import numpy as np
 import pytest
@@ -12,7 +16,7 @@ import pytest
 from jmultislicelib_amd import Engine, JMultiSlice, configs, fft2d
 from oracle import oracle as O
 
-from helpers import random_setup, rel_l2
+from helpers import assert_dets, random_setup, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -65,11 +69,11 @@ def test_engine_single_wave_vs_oracle(ndetper):
     g_int = eng.result(1)
     g_wff = eng.result(16)
     ref, waves = _oracle_passes(sp, [(0.21, 0.37)], var, weights)
-    e_int = rel_l2(g_int, ref[0])
     e_wav = rel_l2(g_wff, waves[0])
-    print(f"ndetper {ndetper}: integrated rel L2 {e_int:.2e}, wave channel {e_wav:.2e}")
+    print(f"ndetper {ndetper}: wave channel {e_wav:.2e}")
     assert g_int.shape == ref[0].shape
-    assert e_int < TOL and e_wav < TOL
+    assert_dets(g_int, ref[0], TOL, f"ndetper {ndetper}")
+    assert e_wav < TOL
 
 
 def test_class_api_dropin_vs_oracle():
@@ -102,17 +106,13 @@ def test_class_api_dropin_vs_oracle():
         res.append(J.GetResult(GPU, 1)[0])
     var = O.variant_sequences(99, sp.nvarslc, sp.objslc, len(pos) * npass)
     ref, _ = _oracle_passes(sp, pos, var, [1.0 / npass] * npass)
-    e = rel_l2(np.array(res), ref[:, 0])
-    print(f"class API: rel L2 {e:.2e}")
-    assert e < TOL
+    assert_dets(np.array(res), ref[:, 0], TOL, "class API")
     # the batched entry gives the same numbers from the same RNG stream
     J.SeedRngEx(99)
     dx = np.array([p[0] for p in pos], np.float32)
     dy = np.array([p[1] for p in pos], np.float32)
     scan = J.GPUMultisliceScan(dx, dy, None, [1.0 / npass] * npass)
-    e2 = rel_l2(scan[:, 0], ref[:, 0])
-    print(f"class API batched scan: rel L2 {e2:.2e}")
-    assert e2 < TOL
+    assert_dets(scan[:, 0], ref[:, 0], TOL, "class API batched scan")
     J.Cleanup()
 
 
@@ -133,9 +133,54 @@ def test_staged_scan_vs_oracle(graph):
     eng.sync()
     g = eng.scan_results()
     ref = O.stem_scan(sp, dx, dy, weights, var, prec=32, nthreads=4)
-    e = rel_l2(g, ref)
-    print(f"staged scan graph={graph}: rel L2 {e:.2e}")
-    assert e < TOL
+    assert_dets(g, ref, TOL, f"staged scan graph={graph}")
+
+
+def test_graph_cache_follows_batch_shape_and_buffers():
+    """Graph replay stays correct when the batch buffers grow between calls
+    and when a scan is staged again with another pass grouping of the same
+    wave count: a graph-on engine gives exactly the graph-off engine's
+    numbers after every call of the sequence (and the oracle's)."""
+    sp = configs.perovskite_setup(n=64, ncell=2, nslices=10, nvar=3)
+    rng = np.random.default_rng(17)
+    engines = [Engine().setup(sp), Engine().setup(sp)]
+    engines[1].set_graph(True)
+
+    def stage(npos, npass, seed):
+        dx = rng.uniform(0, 0.78, npos).astype(np.float32)
+        dy = rng.uniform(0, 0.78, npos).astype(np.float32)
+        w = np.linspace(1.0, 2.0, npass).astype(np.float32)
+        w /= w.sum()
+        var = configs.variant_sequences(sp, npos * npass, seed=seed)
+        for eng in engines:
+            eng.stage_scan(dx, dy, None, w, var)
+        return dx, dy, w, var
+
+    def run(pos0, npos):
+        out = []
+        for eng in engines:
+            eng.run_staged(pos0, npos)
+            eng.sync()
+            out.append(eng.scan_results())
+        assert np.array_equal(out[0], out[1]), f"graph replay differs after run_staged({pos0}, {npos})"
+        return out[0]
+
+    dx, dy, w, var = stage(8, 2, 1)
+    run(0, 2)   # batch of 4 waves: captured
+    run(0, 8)   # 16 waves: the batch buffers grow
+    g = run(0, 2)  # 4 waves again, after the buffers moved
+    ref = O.stem_scan(sp, dx, dy, w, var, prec=32, nthreads=4)
+    assert_dets(g, ref, TOL, "after buffer growth")
+    dx, dy, w, var = stage(4, 4, 2)  # same 16 waves, other grouping and weights
+    g = run(0, 4)
+    ref = O.stem_scan(sp, dx, dy, w, var, prec=32, nthreads=4)
+    assert_dets(g, ref, TOL, "restaged 4 x 4")
+    dx, dy, w, var = stage(2, 8, 3)  # more passes: the weight buffer grows
+    g = run(0, 2)
+    ref = O.stem_scan(sp, dx, dy, w, var, prec=32, nthreads=4)
+    assert_dets(g, ref, TOL, "restaged 2 x 8")
+    for eng in engines:
+        eng.close()
 
 
 @pytest.mark.parametrize("nx,ny", [(128, 64), (64, 128), (360, 384), (96, 160), (2048, 64), (64, 2048),
@@ -152,9 +197,7 @@ def test_nonsquare_mixed_radix(nx, ny):
     eng.multislice(var[1], 0.4, True)
     g = eng.result(1)
     ref, _ = _oracle_passes(sp, [(0.1, 0.05)], var, [0.6, 0.4])
-    e = rel_l2(g, ref[0])
-    print(f"{nx}x{ny}: rel L2 {e:.2e}")
-    assert e < TOL
+    assert_dets(g, ref[0], TOL, f"{nx}x{ny}")
 
 
 def test_single_probe_256_vs_oracle_fp32_fp64():
@@ -169,12 +212,15 @@ def test_single_probe_256_vs_oracle_fp32_fp64():
     win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.0, 0.0)
     r32, w32 = O.multislice(sp, win, var[0], 1.0, prec=32, want_wave=True)
     r64, w64 = O.multislice(sp, win, var[0], 1.0, prec=64, want_wave=True)
-    e_int, e_wav = rel_l2(g_int, r32), rel_l2(g_wav, w32)
-    print(f"256^2x100: gpu vs oracle f32: detectors {e_int:.2e}, exit wave {e_wav:.2e}")
-    print(f"           vs FP64: gpu {rel_l2(g_int, r64):.2e}/{rel_l2(g_wav, w64):.2e}, "
-          f"oracle f32 {rel_l2(r32, r64):.2e}/{rel_l2(w32, w64):.2e}")
     print("detectors gpu", g_int.ravel(), "f64", r64.ravel())
-    assert e_int < TOL and e_wav < TOL
+    assert_dets(g_int, r32, TOL, "256^2x100 gpu vs oracle f32")
+    e_g64 = assert_dets(g_int, r64, TOL, "gpu vs FP64")
+    e_o64 = assert_dets(r32, r64, TOL, "oracle f32 vs FP64")
+    assert np.all(e_g64 <= np.maximum(4.0 * e_o64, 2e-6))
+    e_wav, ew_g64, ew_o64 = rel_l2(g_wav, w32), rel_l2(g_wav, w64), rel_l2(w32, w64)
+    print(f"exit wave: gpu vs oracle f32 {e_wav:.2e}, vs FP64: gpu {ew_g64:.2e}, oracle f32 {ew_o64:.2e}")
+    assert e_wav < TOL and ew_g64 < TOL and ew_o64 < TOL
+    assert ew_g64 <= max(4.0 * ew_o64, 2e-6)
 
 
 def test_full_2048_grid_batch_vs_oracle():
@@ -192,9 +238,7 @@ def test_full_2048_grid_batch_vs_oracle():
     eng.sync()
     g = eng.scan_results()
     ref = O.stem_scan(sp, dx, dy, np.ones(1, np.float32), var, prec=32, nthreads=2)
-    e = rel_l2(g, ref)
-    print(f"2048^2 batch: rel L2 {e:.2e}")
-    assert e < TOL
+    assert_dets(g, ref, TOL, "2048^2 batch")
     eng.close()
 
 
@@ -224,7 +268,7 @@ def test_reference_sample_job_shape_625():
     eng.sync()
     g = eng.scan_results()
     ref = O.stem_scan(sp, dx, dy, weights, var, prec=32, nthreads=8)
-    e = rel_l2(g, ref)
-    print(f"625^2 sample-job shape: {g.shape} results, rel L2 {e:.2e}")
-    assert e < TOL
+    print(f"625^2 sample-job shape: {g.shape} results")
+    assert np.all(ref[:, -1, 2] > 0)  # the 80-200 mrad ring inside the band limit (~164 mrad)
+    assert_dets(g, ref, TOL, "625^2 sample-job shape")
     eng.close()

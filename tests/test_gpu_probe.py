@@ -10,7 +10,7 @@ import pytest
 from jmultislicelib_amd import Engine, JMultiSlice, configs
 from oracle import oracle as O
 
-from helpers import random_setup, rel_l2
+from helpers import assert_dets, random_setup, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -69,7 +69,5 @@ def test_scan_with_device_probe():
             win = O.offset_wave(host_probe, sp.a0[0], sp.a0[1], sp.wl, dx[p], dy[p])
             res, _ = O.multislice(sp, win, var[p * npass + r], w[r], prec=32)
             ref[p] += res
-    e = rel_l2(got, ref)
-    print(f"scan with device probe: rel L2 {e:.2e}")
-    assert e < 1e-4
+    assert_dets(got, ref, 1e-4, "scan with device probe")
     eng.close()

@@ -120,6 +120,85 @@ def test_probe_norm_and_aperture():
     assert np.all(np.abs(p.imag) == 0)
 
 
+def _aperture_closed_form(nx, ny, ax, ay, wl, alpha, rs, asym, adir_deg, x0, y0, btx, bty):
+    """Probe amplitude in float64 from the geometry: an aperture of radius
+    qlim = alpha/wl centred at (x0, y0)/wl, stretched by (1 + asym) along the
+    direction adir and by (1 - asym) across it, with the edge profile
+    (1 - tanh(pi (|u| - qlim) / (rs qlim))) / 2 in the undistorted radius |u|
+    (hard edge for rs = 0), tilted by (btx, bty)/wl, amplitudes <= 1e-6 cut,
+    normalised to unit intensity (JProbeGen.cpp:446-505, 707-777)."""
+    nx2, ny2 = nx // 2, ny // 2
+    qx = ((np.arange(nx) + nx2) % nx - nx2) / ax
+    qy = ((np.arange(ny) + ny2) % ny - ny2) / ay
+    qlim = alpha * 1e-3 / wl
+    dqx = qx[None, :] - btx * 1e-3 / wl - x0 * 1e-3 / wl
+    dqy = qy[:, None] - bty * 1e-3 / wl - y0 * 1e-3 / wl
+    phi = np.deg2rad(adir_deg)
+    along = dqx * np.cos(phi) + dqy * np.sin(phi)
+    across = -dqx * np.sin(phi) + dqy * np.cos(phi)
+    u = np.hypot(along / (1.0 + asym), across / (1.0 - asym))
+    if rs > 0:
+        a = 0.5 * (1.0 - np.tanh(np.pi * (u - qlim) / (rs * qlim)))
+    else:
+        a = (u < qlim).astype(np.float64)
+    a[u == 0] = 1.0
+    a[a <= 1e-6] = 0.0
+    return a / np.sqrt(np.sum(a * a))
+
+
+@pytest.mark.parametrize("alpha,rs,asym,adir,x0,y0,btx,bty", [
+    (25.0, 0.03, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+    (22.0, 0.12, 0.0, 0.0, 1.5, -2.0, 0.0, 0.0),
+    (20.0, 0.08, 0.15, 30.0, 0.0, 0.0, 0.0, 0.0),
+    (24.0, 0.05, 0.25, -65.0, -1.0, 2.5, 3.0, -1.5),
+    (18.0, 0.0, 0.2, 110.0, 0.0, 0.0, 0.0, 0.0),
+])
+def test_probe_aperture_closed_form(alpha, rs, asym, adir, x0, y0, btx, bty):
+    """Smooth edge, asymmetry, aperture shift and beam tilt of the probe
+    aperture against the geometric closed form: the amplitude ratio of any
+    two pixels is the ratio of the edge profile at their undistorted radii.
+    A sigmoid without pi, a smoothing width not scaled by qlim, a transposed
+    or sign-flipped back-projection, an undoubled angle or a tilt applied
+    twice each fail it."""
+    nx, ny, ax, ay, wl = 128, 96, 3.124, 2.343, 0.0025079
+    p = O.probe_wave(nx, ny, ax, ay, wl, alpha, rs, asym, adir, x0, y0, btx, bty)
+    want = _aperture_closed_form(nx, ny, ax, ay, wl, alpha, rs, asym, adir, x0, y0, btx, bty)
+    assert np.all(p.imag == 0)
+    got = np.abs(p).astype(np.float64)
+    if rs > 0:
+        assert np.count_nonzero((want > 0.05 * want.max()) & (want < 0.95 * want.max())) > 20  # edge pixels
+        assert np.max(np.abs(got - want)) < 2e-5 * want.max()
+    else:  # hard edge: pixels exactly on the rim may round either way in float32
+        diff = (got > 0) != (want > 0)
+        assert diff.sum() <= 2
+        assert np.max(np.abs(got - want)[~diff]) < 1e-5 * want.max()
+
+
+@pytest.mark.parametrize("nprotype", [1, 0])
+def test_radial_profile_sensitivity_closed_form(nprotype):
+    """A linear sensitivity profile p[i] = c0 + c1 i is reproduced exactly by
+    linear interpolation: on the ring the detector reads
+    c0 + c1 min(g (refpix - 1) / g1, len - 1), where g is the pixel's |k| and
+    g1 the inner edge in the same units (JMultiSlice.cpp:1247-1272,
+    1407-1416)."""
+    nx, ny, a0 = 128, 128, 3.124
+    wl = 0.0025079
+    c0, c1, plen, refpix = 0.25, 0.03, 30, 9.0
+    prof = (c0 + c1 * np.arange(plen)).astype(np.float32)
+    b0, b1 = 20.0, 95.0
+    det, msk = O.ring_detector(nx, ny, a0, a0, wl, nprotype, b0, b1, profile=prof, refpix=refpix)
+    plain, pm = O.ring_detector(nx, ny, a0, a0, wl, nprotype, b0, b1)
+    assert np.array_equal(msk, pm)
+    k2, _, _ = _k2(nx, ny, a0, a0)
+    g = np.sqrt(k2)
+    g1 = b0 * 1e-3 / wl if nprotype == 1 else np.sin(b0 * 1e-3) / wl
+    want = c0 + c1 * np.minimum(g * (refpix - 1.0) / g1, plen - 1)
+    on = plain > 0
+    assert on.sum() > 1000 and np.any(g[on] * (refpix - 1.0) / g1 > plen - 1)  # the clamp is reached
+    assert np.allclose(det[on], want[on], rtol=2e-6, atol=0)
+    assert np.all(det[~on] == 0)
+
+
 @pytest.mark.parametrize("ndetper,nobj,expect", [
     (0, 10, {10: 0}),
     (3, 10, {0: 0, 3: 1, 6: 2, 9: 3, 10: 4}),

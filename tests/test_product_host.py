@@ -42,6 +42,31 @@ def test_abi_exports_every_declared_symbol(header):
     assert not sorted(declared - set(J.EXPORTED))
 
 
+def test_every_reference_export_resolves():
+    """Every entry point the reference exports from its flat C-ABI
+    (src/JMultiSliceLib.h:111-380; names listed in
+    tests/golden/reference_exports.txt) resolves in the library."""
+    import subprocess
+    names_file = os.path.join(os.path.dirname(__file__), "golden", "reference_exports.txt")
+    names = [l.strip() for l in open(names_file) if l.strip()]
+    assert len(names) == 57
+    out = subprocess.run(["nm", "-D", "--defined-only", J.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert not sorted(set(names) - exported)
+
+
+def test_cpu_path_entries_return_error():
+    """The CPU-only entries exist and refuse (code 1): this library has no
+    CPU multislice path (CPUMultislice :333, SetIncomingWaveCPU :293,
+    ClearDetMem_h :357).  SetPlasmonMC is accepted and switched off again."""
+    L = J.lib()
+    w = np.zeros(2 * 16, np.float32)
+    assert L.CPUMultislice(0, 0, 1.0, 0) == 1
+    assert L.SetIncomingWaveCPU(w.ctypes.data, 0, 0) == 1
+    assert L.ClearDetMem_h(0) == 1
+    L.SetPlasmonMC(0, 0.0, 0.0, 0.0, 0)
+
+
 def test_library_loads():
     L = J.lib()
     assert abs(L.GetJMSVersion() - 0.46) < 1e-6

@@ -74,3 +74,200 @@ def test_frozen_phonon_draws_match_reference_rng():
     L.ref_rng_destroy(r)
     expect = xy0 + 0.08 * d  # the library adds rms * normrand(0, 1) in double
     assert np.array_equal(xy, expect)
+
+
+# ---------------------------------------------------------------------------
+# GetRandomVariantSequence (JMultiSlice.cpp:1571-1676) driven by the
+# reference generator.  The procedure, per object slice of a structure slice
+# type with nv >= 2 variants: one draw u = unirand(0, nv) of the shared
+# generator, k = int(u) mod nv, then the first index k, k+1, ... (cyclic) that
+# is still unused in the type's current window of nv draws and differs from
+# the type's previous draw; a window that is used up starts again full.
+# Types with one variant give 0 and draw nothing.
+# ---------------------------------------------------------------------------
+
+
+def _sequence_model(draw, nvarslc, objslc, step=1, draws_per_slice=1):
+    """The procedure above as a window of availability flags; draw(nv) gives
+    the next uniform deviate on [0, nv].  step / draws_per_slice let a test
+    show that a backward scan or an extra draw would change the result."""
+    avail = {t: [] for t in range(len(nvarslc))}
+    last = {t: -1 for t in range(len(nvarslc))}
+    out = []
+    for t in objslc:
+        nv = nvarslc[t]
+        if nv < 2:
+            out.append(0)
+            continue
+        if not any(avail[t]):
+            avail[t] = [True] * nv
+        for _ in range(draws_per_slice):
+            k = int(draw(nv)) % nv
+        k0 = k
+        while not (avail[t][k] and k != last[t]):
+            k = (k + step) % nv
+            assert k != k0, "no valid index left (cannot happen for nv >= 2)"
+        avail[t][k] = False
+        last[t] = k
+        out.append(k)
+    return np.array(out, np.int32)
+
+
+class _RefStream:
+    """Uniform deviates of the reference CRng, drawn in blocks on [0, 1] and
+    scaled: unirand(0, nv) = 0 + (rand()/MAX) * (nv - 0) is the same double
+    product as nv * unirand(0, 1) (checked in test_block_draws_scale)."""
+
+    def __init__(self, L, seed, oub=False):
+        self.L, self.r, self.oub = L, L.ref_rng_create(seed), oub
+        self.buf = np.zeros(4096, np.float64)
+        self.i = len(self.buf)
+
+    def __call__(self, nv):
+        if self.i == len(self.buf):
+            self.L.ref_rng_unirand_n(self.r, 0.0, 1.0, len(self.buf), int(self.oub),
+                                     self.buf.ctypes.data_as(ctypes.c_void_p))
+            self.i = 0
+        v = self.buf[self.i] * float(nv)
+        self.i += 1
+        return v
+
+    def close(self):
+        self.L.ref_rng_destroy(self.r)
+
+
+def _ref_n():
+    L = _ref()
+    L.ref_rng_unirand_n.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_void_p]
+    return L
+
+
+def _oracle_sequence(seed, nvarslc, objslc):
+    """The oracle's sequence and its generator (left where the sequence ended)."""
+    r = O.Rng(seed)
+    nv = np.ascontiguousarray(nvarslc, np.int32)
+    ob = np.ascontiguousarray(objslc, np.int32)
+    out = np.zeros(len(ob), np.int32)
+    assert O.L().orc_variant_sequence(r.h, len(nv), O._p(nv), len(ob), O._p(ob), O._p(out)) == 0
+    return out, r
+
+
+def test_block_draws_scale():
+    """nv * unirand(0, 1) equals unirand(0, nv) of the same draw, bit for bit
+    (also for unirand_oub), so the block stream reproduces single draws."""
+    L = _ref_n()
+    for oub in (False, True):
+        for nv in (2, 3, 8, 4096):
+            single = L.ref_rng_create(11)
+            f = L.ref_rng_unirand_oub if oub else L.ref_rng_unirand
+            want = [f(single, 0.0, float(nv)) for _ in range(5000)]
+            L.ref_rng_destroy(single)
+            st = _RefStream(L, 11, oub)
+            got = [st(nv) for _ in range(5000)]
+            st.close()
+            assert got == want
+
+
+NV_MIX = [1, 2, 3, 8, 4096]
+
+
+def _mixed_objslc(seed, n=20000):
+    return np.random.default_rng(seed).integers(0, len(NV_MIX), n).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", [1, 42, 977, 31337])
+def test_variant_sequence_follows_reference_draws(seed):
+    """The oracle's and the library's sequences equal the procedure driven by
+    the reference generator's own draws, over 20000 object slices of five
+    slice types (1, 2, 3, 8 and 4096 variants), and the oracle's generator
+    ends exactly one draw per multi-variant object slice further on."""
+    from jmultislicelib_amd._lib import lib, ptr
+    L = _ref_n()
+    obj = _mixed_objslc(seed)
+    st = _RefStream(L, seed)
+    want = _sequence_model(st, NV_MIX, obj)
+    st.close()
+    got, orng = _oracle_sequence(seed, NV_MIX, obj)
+    assert np.array_equal(got, want)
+    nv = np.ascontiguousarray(NV_MIX, np.int32)
+    lib_out = np.zeros((1, len(obj)), np.int32)
+    assert lib().jms_variant_sequences(seed, len(nv), ptr(nv), len(obj), ptr(obj), 1, ptr(lib_out)) == 0
+    assert np.array_equal(lib_out[0], want)
+    # generator position: one draw per object slice with >= 2 variants
+    ndraw = int(np.sum(np.asarray(NV_MIX)[obj] >= 2))
+    r = L.ref_rng_create(seed)
+    skip = np.zeros(ndraw, np.float64)
+    L.ref_rng_unirand_n(r, 0.0, 1.0, ndraw, 0, skip.ctypes.data_as(ctypes.c_void_p))
+    assert [L.ref_rng_rand(r) for _ in range(8)] == [orng.rand() for _ in range(8)]
+    L.ref_rng_destroy(r)
+    # the check is sensitive to a backward scan and to a second draw per slice
+    for kw in ({"step": -1}, {"draws_per_slice": 2}):
+        st = _RefStream(L, seed)
+        other = _sequence_model(st, NV_MIX, obj, **kw)
+        st.close()
+        assert not np.array_equal(other, want), kw
+
+
+@pytest.mark.parametrize("seed", [3, 8, 2024])
+def test_two_variants_alternate_from_first_draw(seed):
+    """With two variants the sequence is forced: a, 1-a, a, ... with
+    a = int(unirand(0, 2)) mod 2 of the first draw."""
+    L = _ref_n()
+    r = L.ref_rng_create(seed)
+    a = int(L.ref_rng_unirand(r, 0.0, 2.0)) % 2
+    L.ref_rng_destroy(r)
+    obj = np.zeros(41, np.int32)
+    got, _ = _oracle_sequence(seed, [2], obj)
+    assert np.array_equal(got, (a + np.arange(41)) % 2)
+
+
+def test_window_starts_take_the_drawn_index():
+    """At the start of every window of nv draws the drawn index itself is
+    taken unless it repeats the previous draw; otherwise the next index up."""
+    L = _ref_n()
+    nv, n = 8, 8 * 400
+    obj = np.zeros(n, np.int32)
+    got, _ = _oracle_sequence(77, [nv], obj)
+    st = _RefStream(L, 77)
+    k = np.array([int(st(nv)) % nv for _ in range(n)])
+    st.close()
+    starts = np.arange(0, n, nv)
+    prev = np.concatenate([[-1], got[starts[1:] - 1]])
+    fresh = k[starts] != prev
+    assert fresh.sum() > 300 and (~fresh).sum() > 10
+    assert np.array_equal(got[starts][fresh], k[starts][fresh])
+    assert np.array_equal(got[starts][~fresh], (k[starts][~fresh] + 1) % nv)
+
+
+def test_unirand_not_unirand_oub():
+    """The draw is unirand (scale 1/MAX), not unirand_oub (1/(MAX+1)): with
+    3001 variants they give different indices for about one draw in 1.4
+    million (with a power of two they differ only at rand() = MAX).  The
+    first seed whose sequence of 9003 draws contains such a draw, with a
+    different outcome, must follow unirand."""
+    L = _ref_n()
+    nv, n = 3001, 9003
+    obj = np.zeros(n, np.int32)
+    u = np.zeros(n, np.float64)
+    o = np.zeros(n, np.float64)
+    found = None
+    for seed in range(1, 4000):
+        for arr, oub in ((u, 0), (o, 1)):
+            r = L.ref_rng_create(seed)
+            L.ref_rng_unirand_n(r, 0.0, 1.0, n, oub, arr.ctypes.data_as(ctypes.c_void_p))
+            L.ref_rng_destroy(r)
+        if np.any((u * nv).astype(np.int64) % nv != (o * nv).astype(np.int64) % nv):
+            st = _RefStream(L, seed)
+            want = _sequence_model(st, [nv], obj)
+            st.close()
+            st = _RefStream(L, seed, oub=True)
+            alt = _sequence_model(st, [nv], obj)
+            st.close()
+            if not np.array_equal(want, alt):
+                found = (seed, want)
+                break
+    assert found is not None
+    seed, want = found
+    got, _ = _oracle_sequence(seed, [nv], obj)
+    assert np.array_equal(got, want)
