@@ -10,16 +10,24 @@ passes); the default 8 timed steps cover the full 4096-position scan.
 All inputs (gratings, propagator, detectors, probe, positions, variant
 sequences) are resident in HBM before the timed region.
 
-Prints ONE JSON line.  Kernel durations come from HIP events recorded around
-every launch on the engine's stream inside the timed region; the roofline
-object reports the dominant kernel's algorithmic bytes (16 * n per wave: one
-read and one write of the complex64 wave) per average launch duration against
-MI355X's 8 TB/s HBM peak; its `traffic` is the HBM bytes per launch measured
-with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size)
-and committed in profiles/hbm_traffic_stem_scan_512.md, per wave times the
-waves per launch.  cpu_baseline times the CPU oracle (this project's
-restatement of the reference CPUMultislice, float32, one std::thread per
-core) on a bounded sample of the same workload.
+Prints ONE JSON line.  The timed region replays each batch's captured HIP
+graph (the batch's whole launch sequence; --no-graph: eager launches), with
+HIP events around every replay.  A second pass over the same steps with eager
+launches and HIP events around every kernel launch (on the engine's stream)
+gives the per-kernel durations behind `roofline`: the dominant kernel's
+algorithmic bytes per launch (16 n per wave: one read and one write of the
+complex64 wave; 8 n for the initial column pass, which reads the shared
+probe) over its average launch duration, against MI355X's 8 TB/s HBM peak.
+Its `traffic` is the HBM bytes per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size) and committed
+in profiles/hbm_traffic_stem_scan_512.md, per wave times the waves per
+launch.  `fft_rate` is 10 n log2 n flops per probe pass and slice over the
+timed region against the FP32 vector peak.  Outside the timed region,
+`accuracy` compares a subset of the scan (8 positions x 8 passes) with the
+CPU oracle's float32 run and its FP64 twin, detector by detector, and
+`cpu_baseline` is that float32 oracle run (this project's restatement of the
+reference CPUMultislice, one std::thread per core), with the CPU model and a
+one-thread figure.
 """
 from __future__ import annotations
 
@@ -57,19 +65,65 @@ def measured_traffic_per_wave(kernel):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(sp, dx, dy, var, npass, cores):
-    """Oracle float32 scan on `cores` threads: 12*cores probe passes (one
-    position each, about 10 s of CPU work), scaled to positions/s of npass
-    passes."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# raster indices of the accuracy subset of the 64x64 scan: the A column at
+# the origin, points between columns, a B column, the far corner
+ACC_IDX = np.array([0, 37, 500, 1056, 1500, 2080, 3000, 4095])
+
+
+def accuracy_and_cpu_baseline(eng, sp, dx_all, dy_all, npass, cores):
+    """Outside the timed region.  On the accuracy subset (8 positions x
+    npass passes of the headline scan, own variant sequences from seed 42):
+    the GPU result, the oracle's float32 run (timed: the CPU baseline, on
+    `cores` threads) and its FP64 twin; per-detector relative L2 errors of
+    GPU vs oracle f32, GPU vs FP64 and oracle f32 vs FP64.  Then one probe
+    pass of the oracle on one thread (the single-core figure)."""
+    from jmultislicelib_amd import configs
     from oracle import oracle as O
-    nsample = 12 * cores
+    dx, dy = dx_all[ACC_IDX], dy_all[ACC_IDX]
+    nacc = len(ACC_IDX)
+    w = np.full(npass, 1.0 / npass, np.float32)
+    var = configs.variant_sequences(sp, nacc * npass, seed=42)
+    eng.set_graph(False)
+    eng.set_timing(False)
+    eng.stage_scan(dx, dy, None, w, var)
+    eng.run_staged(0, nacc)
+    eng.sync()
+    g = eng.scan_results()[:, 0, :].astype(np.float64)
     t0 = time.perf_counter()
-    O.stem_scan(sp, dx[:nsample], dy[:nsample], [1.0], var[:nsample], prec=32, nthreads=cores)
+    r32 = O.stem_scan(sp, dx, dy, w, var, prec=32, nthreads=cores)[:, 0, :].astype(np.float64)
     dt = time.perf_counter() - t0
-    passes_per_s = nsample / dt
-    return {"value": passes_per_s / npass, "unit": "positions/s", "cores": cores, "kind": "port",
-            "sample": f"{nsample} probe passes (one per position, 512^2, 200 slices, 3 detectors) "
-                      f"in {dt:.1f} s on {cores} threads; positions/s = passes/s / {npass}"}
+    r64 = O.stem_scan(sp, dx, dy, w, var, prec=64, nthreads=cores)[:, 0, :].astype(np.float64)
+    t1 = time.perf_counter()
+    O.stem_scan(sp, dx[:1], dy[:1], [1.0], var[:1], prec=32, nthreads=1)
+    dt1 = time.perf_counter() - t1
+
+    def rel(a, b):
+        return [float(np.linalg.norm(a[:, d] - b[:, d]) / np.linalg.norm(b[:, d])) for d in range(a.shape[1])]
+
+    names = ["BF 0-11 mrad", "ABF 11-22 mrad", "HAADF 70-200 mrad"]
+    acc = {"subset": f"{nacc} positions (raster indices {ACC_IDX.tolist()}) x {npass} passes, exit plane",
+           "detectors": names,
+           "gpu_vs_oracle_f32": [float(f"{e:.3e}") for e in rel(g, r32)],
+           "gpu_vs_fp64": [float(f"{e:.3e}") for e in rel(g, r64)],
+           "oracle_f32_vs_fp64": [float(f"{e:.3e}") for e in rel(r32, r64)],
+           "tolerance": 1e-4}
+    cpu = {"value": round(nacc / dt, 4), "unit": "positions/s", "cores": cores, "kind": "port",
+           "sample": f"oracle float32 scan of the accuracy subset: {nacc} positions x {npass} passes "
+                     f"(512^2, 200 slices, 3 detectors) in {dt:.1f} s on {cores} threads",
+           "cpu_model": cpu_model(),
+           "one_thread": {"value": round(1.0 / (dt1 * npass), 5), "unit": "positions/s",
+                          "sample": f"1 probe pass in {dt1:.2f} s on 1 thread; positions/s = passes/s / {npass}"}}
+    return acc, cpu
 
 
 def run_thick2048(args):
@@ -272,6 +326,141 @@ def run_surrogate(args):
     print(json.dumps(out_json), flush=True)
 
 
+def run_stem512(args):
+    """The headline scan (config: 512^2 x 200 slices, 8 frozen-phonon passes
+    per position).  Timed region: `steps` steps of positions_per_step
+    positions, each batch replayed from its captured HIP graph (--no-graph:
+    eager launches), HIP events around every replay.  Then, untimed for
+    `value`, the same steps with eager launches and HIP events around every
+    kernel launch: the per-kernel durations behind `roofline`."""
+    from jmultislicelib_amd import Engine, configs
+
+    npass = 8
+    graph = not args.no_graph
+    t0 = time.perf_counter()
+    sp = configs.stem_scan_512()
+    dx_all, dy_all = configs.scan_positions(64, 2)
+    npos_total = len(dx_all)
+    pps = args.positions_per_step
+    nsteps_needed = args.warmup + args.steps
+    # position p of step s: the scan is walked cyclically
+    order = np.arange(nsteps_needed * pps) % npos_total
+    dx, dy = dx_all[order], dy_all[order]
+    var = configs.variant_sequences(sp, len(order) * npass, seed=42)
+    log(f"setup {time.perf_counter() - t0:.1f} s: {sp.nx}x{sp.ny}, {len(sp.objslc)} slices, "
+        f"{len(order)} positions x {npass} passes")
+
+    eng = Engine().setup(sp)
+    eng.set_batch(args.batch)
+    weights = np.full(npass, 1.0 / npass, np.float32)
+    eng.stage_scan(dx, dy, None, weights, var)
+    eng.sync()
+
+    def timed_steps(use_graph):
+        eng.set_graph(use_graph)
+        eng.set_timing(False)
+        for s_ in range(args.warmup):  # captures the batch graphs when use_graph
+            eng.run_staged(s_ * pps, pps)
+        eng.sync()
+        eng.set_timing(True)
+        t1 = time.perf_counter()
+        for s_ in range(args.warmup, nsteps_needed):
+            eng.run_staged(s_ * pps, pps)
+        eng.sync()
+        return time.perf_counter() - t1
+
+    wall = timed_steps(graph)
+    res = eng.scan_results()
+    finite = bool(np.isfinite(res).all())
+    n = sp.nx * sp.ny
+    S = len(sp.objslc)
+    positions = args.steps * pps
+    value = positions / wall
+    replay = None
+    if graph:
+        ms, cnt = eng.timing(3)
+        replay = {"replays": cnt, "avg_replay_ms": round(ms / max(cnt, 1), 3),
+                  "waves_per_replay": args.batch, "gpu_ms_total": round(ms, 1)}
+        log(f"graph replays: {cnt}, {ms:.1f} ms total, {ms / max(cnt, 1):.2f} ms avg")
+        wall_eager = timed_steps(False)
+        log(f"eager launches (kernel timing pass): {positions / wall_eager:.1f} positions/s")
+    else:
+        wall_eager = wall
+    kinds = {0: "row_pass", 1: "col_pass", 2: "combine"}
+    tm = {k: eng.timing(k) for k in kinds}
+    for k, (ms, cnt) in tm.items():
+        log(f"kernel {kinds[k]}: {cnt} launches, {ms:.1f} ms total, {ms / max(cnt, 1):.3f} ms avg")
+    dom = max((0, 1), key=lambda k: tm[k][0])
+    ms, cnt = tm[dom]
+    passes = positions * npass
+    if dom == 0:  # S row passes per wave, 16 n bytes each
+        total_bytes = 16.0 * n * passes * S
+    else:  # S column passes of 16 n bytes plus the initial one: probe read shared, 8 n written
+        total_bytes = (16.0 * n * S + 8.0 * n) * passes
+    bytes_per_launch = total_bytes / max(cnt, 1)
+    achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
+    nwave_avg = passes * (S + (1 if dom == 1 else 0)) / max(cnt, 1)
+    tpw = measured_traffic_per_wave(kinds[dom])
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if tpw is None else round(tpw * nwave_avg),
+            "traffic_source": None if tpw is None else os.path.relpath(TRAFFIC_PROFILE, ROOT),
+            "kernel": kinds[dom], "bytes_per_launch": round(bytes_per_launch),
+            "avg_launch_ms": round(ms / cnt, 4),
+            "timing": "HIP events on the engine stream around every eager launch"}
+    total_kernel_ms = sum(v[0] for v in tm.values())
+    log(f"kernel time {total_kernel_ms / 1e3:.2f} s of {wall_eager:.2f} s wall (eager pass)")
+    # whole-slice view: 32 n bytes per probe pass per slice
+    slice_bytes = 32.0 * n * passes * S
+    flops = 10.0 * n * np.log2(n) * passes * S  # two 2-D FFTs of 5 n log2 n per probe pass and slice
+    log(f"scan: {positions} positions in {wall:.2f} s -> {value:.1f} positions/s; "
+        f"slice-loop HBM rate {slice_bytes / wall / 1e9:.0f} GB/s ({slice_bytes / wall / 1e9 / HBM_PEAK_GBS:.1%} of peak); "
+        f"FFT rate {flops / wall / 1e12:.1f} TFLOP/s")
+    log(f"results finite: {finite}; first position detectors {res[0, 0]}")
+
+    acc = cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cores = min(16, len(os.sched_getaffinity(0)))
+            acc, cpu = accuracy_and_cpu_baseline(eng, sp, dx_all, dy_all, npass, cores)
+            log(f"accuracy: {acc}")
+            log(f"cpu baseline: {cpu['value']:.3f} positions/s ({cpu['sample']}); 1 thread "
+                f"{cpu['one_thread']['value']:.4f} positions/s; {cpu['cpu_model']}")
+        except Exception as ex:  # the baseline must not hide the GPU result
+            log(f"accuracy / cpu baseline failed: {ex}")
+    eng.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "positions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "stem_scan_512: 512x512 complex64, 200 slices (2 types x 8 frozen-phonon "
+                               "variants), 64x64 positions x 8 passes, 3 annular detectors",
+                   "grid": [sp.nx, sp.ny], "slices": int(S), "positions_per_step": pps,
+                   "passes": npass, "batch_waves": args.batch, "graph": graph,
+                   "graph_replays": replay,
+                   "eager_positions_per_s": round(positions / wall_eager, 2),
+                   "slice_loop_hbm_frac": round(slice_bytes / wall / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": roof,
+        "fft_rate": {"achieved": round(flops / wall / 1e12, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(flops / wall / 1e12 / 157.3, 4),
+                     "flops_per_probe_pass_slice": int(10 * n * np.log2(n)),
+                     "note": "10 n log2 n per probe pass and slice over the timed region; FP32 vector peak"},
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,7 +469,9 @@ def main():
     ap.add_argument("--positions-per-step", type=int, default=512)
     ap.add_argument("--batch", type=int, default=2048, help="probe passes resident per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="replay batches from a HIP graph (no per-kernel timing)")
+    ap.add_argument("--graph", action="store_true", help="jobs64: replay batches from HIP graphs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="stem512: time eager launches instead of HIP graph replays")
     ap.add_argument("--workload", choices=["stem512", "thick2048", "jobs64", "surrogate"], default="stem512",
                     help="stem512: the headline scan (default); thick2048: 2048^2 x 1000 distinct slices; "
                          "jobs64: uneven batch of 64 random-crystal jobs (edges 360-720); surrogate: "
@@ -322,101 +513,7 @@ def main():
         run_surrogate(args)
         return
 
-    from jmultislicelib_amd import Engine, configs
-
-    npass = 8
-    t0 = time.perf_counter()
-    sp = configs.stem_scan_512()
-    dx_all, dy_all = configs.scan_positions(64, 2)
-    npos_total = len(dx_all)
-    pps = args.positions_per_step
-    nsteps_needed = args.warmup + args.steps
-    # position p of step s: the scan is walked cyclically
-    order = np.arange(nsteps_needed * pps) % npos_total
-    dx, dy = dx_all[order], dy_all[order]
-    var = configs.variant_sequences(sp, len(order) * npass, seed=42)
-    log(f"setup {time.perf_counter() - t0:.1f} s: {sp.nx}x{sp.ny}, {len(sp.objslc)} slices, "
-        f"{len(order)} positions x {npass} passes")
-
-    eng = Engine().setup(sp)
-    eng.set_batch(args.batch)
-    eng.set_graph(args.graph)
-    weights = np.full(npass, 1.0 / npass, np.float32)
-    eng.stage_scan(dx, dy, None, weights, var)
-    eng.sync()
-
-    for s in range(args.warmup):
-        eng.run_staged(s * pps, pps)
-    eng.sync()
-    eng.set_timing(not args.graph)
-    t1 = time.perf_counter()
-    for s in range(args.warmup, nsteps_needed):
-        eng.run_staged(s * pps, pps)
-    eng.sync()
-    t2 = time.perf_counter()
-    wall = t2 - t1
-    res = eng.scan_results()
-    finite = bool(np.isfinite(res).all())
-    n = sp.nx * sp.ny
-    positions = args.steps * pps
-    value = positions / wall
-    roof = None
-    if not args.graph:
-        kinds = {0: "row_pass", 1: "col_pass", 2: "combine"}
-        tm = {k: eng.timing(k) for k in kinds}
-        for k, (ms, cnt) in tm.items():
-            log(f"kernel {kinds[k]}: {cnt} launches, {ms:.1f} ms total, {ms / max(cnt, 1):.3f} ms avg")
-        dom = max((0, 1), key=lambda k: tm[k][0])
-        ms, cnt = tm[dom]
-        nwave_avg = positions * npass * (len(sp.objslc) + (1 if dom == 1 else 0)) / max(cnt, 1)
-        bytes_per_launch = 16.0 * n * nwave_avg
-        achieved = bytes_per_launch / (ms / cnt * 1e-3) / 1e9
-        tpw = measured_traffic_per_wave(kinds[dom]) if sp.nx == 512 and sp.ny == 512 else None
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None if tpw is None else round(tpw * nwave_avg),
-                "traffic_source": None if tpw is None else os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                "kernel": kinds[dom], "bytes_per_launch": bytes_per_launch,
-                "avg_launch_ms": round(ms / cnt, 4)}
-        total_kernel_ms = sum(v[0] for v in tm.values())
-        log(f"kernel time {total_kernel_ms / 1e3:.2f} s of {wall:.2f} s wall")
-    # whole-slice view: 32 n bytes per probe pass per slice
-    slice_bytes = 32.0 * n * positions * npass * len(sp.objslc)
-    log(f"scan: {positions} positions in {wall:.2f} s -> {value:.1f} positions/s; "
-        f"slice-loop HBM rate {slice_bytes / wall / 1e9:.0f} GB/s ({slice_bytes / wall / 1e9 / HBM_PEAK_GBS:.1%} of peak)")
-    log(f"results finite: {finite}; first position detectors {res[0, 0]}")
-    eng.close()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cores = min(16, len(os.sched_getaffinity(0)))
-            cpu = cpu_baseline(sp, dx_all, dy_all, var, npass, cores)
-            log(f"cpu baseline: {cpu['value']:.3f} positions/s ({cpu['sample']})")
-        except Exception as ex:  # the baseline must not hide the GPU result
-            log(f"cpu baseline failed: {ex}")
-
-    out = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "positions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "stem_scan_512: 512x512 complex64, 200 slices (2 types x 8 frozen-phonon "
-                               "variants), 64x64 positions x 8 passes, 3 annular detectors",
-                   "grid": [sp.nx, sp.ny], "slices": int(len(sp.objslc)), "positions_per_step": pps,
-                   "passes": npass, "batch_waves": args.batch, "graph": bool(args.graph)},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    run_stem512(args)
 
 
 if __name__ == "__main__":

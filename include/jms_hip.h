@@ -176,9 +176,10 @@ int jms_hip_readout_det_avg(jms_hip_engine* e, int which, float* dst, float* wei
 /* in-place 2-D FFT of nbatch host arrays (dir < 0: forward exp(-i), dir > 0:
  * backward exp(+i), unscaled) on the device -- JFFTCUDAcore::FT/IFT :299-336 */
 int jms_hip_fft2d(int nx, int ny, int nbatch, float* data, int dir);
-/* kernel timing with HIP events on the engine stream (eager launches only):
- * enable, then read the summed duration [ms] and launch count per kernel kind
- * (0 = row pass, 1 = column pass, 2 = combine) */
+/* timing with HIP events on the engine stream: enable, then read the summed
+ * duration [ms] and count per kind (0 = row pass, 1 = column pass,
+ * 2 = combine: eager launches; 3 = one replay of a batch's HIP graph, which
+ * runs the batch's whole launch sequence) */
 int jms_hip_set_timing(jms_hip_engine* e, int enable);
 int jms_hip_get_timing(jms_hip_engine* e, int kind, double* ms, long long* count);
 /* last HIP error text of the engine (static storage) */

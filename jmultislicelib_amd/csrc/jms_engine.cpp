@@ -105,7 +105,7 @@ struct jms_hip_engine {
   std::array<const void*, 16> graph_bufs{};
   // timing
   bool timing = false;
-  Timer tm[3];
+  Timer tm[4];  // row pass, column pass, combine, graph replay
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_next = 0;
@@ -215,18 +215,19 @@ struct BatchChans {
   bool real_space() const { return img || img_avg || wfr || wfr_avg; }
 };
 
-// Launch sequence of one batch.  `ppar` (float4 per wave) and `var` (nobjslc
-// ints per wave) describe the waves; src/src_stride the INIT source.
-// Readout partial sums go to e->partial, combined into `res` (npos results)
-// when npass > 0; `ch` are the single-wave channels.
-int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long long src_stride,
-                  const void* ppar, const int* var, int npos, int npass, const float* weights,
-                  float* res, int accumulate, const BatchChans& ch, bool timed) {
+// Launch sequence of one batch.  `waves` holds the nwave working waves
+// (stride n); `ppar` (float4 per wave) and `var` (nobjslc ints per wave)
+// describe them; src/src_stride the INIT source.  Readout partial sums go to
+// `partial` ([wave][plane][detector][column tile]), combined into `res`
+// (npos results) when npass > 0; `ch` are the single-wave channels.
+int enqueue_batch(jms_hip_engine* e, void* waves, float* partial, int nwave, int islc0, const void* src,
+                  long long src_stride, const void* ppar, const int* var, int npos, int npass,
+                  const float* weights, float* res, int accumulate, const BatchChans& ch, bool timed) {
   const int S = e->nobjslc;
   const int ntc = e->ntiles_col();
   const long long part_w = (long long)e->ndetslc * e->ndet * ntc;
   jms::ColArgsHost ca;
-  ca.wave = e->waves.p;
+  ca.wave = waves;
   ca.nx = e->nx;
   ca.wave_stride = e->n;
   ca.kscx = 1.f / e->a0x;
@@ -236,7 +237,7 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   ca.detw_t = e->detw_t.as<float>();
   ca.ndet = (e->detflags & JMS_HIP_DETECT_INTEGRATED) ? e->ndet : 0;
   ca.detw = e->detw.as<float>();
-  ca.partial = e->partial.as<float>();
+  ca.partial = partial;
   ca.part_wstride = part_w;
   ca.part_dstride = ntc;
   ca.dif = ch.dif;
@@ -283,7 +284,7 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
       const int mode = (id >= 0 && ch.real_space()) ? 1 : 0;
       rc.chan_off = id >= 0 ? (long long)id * e->n : 0;
       TimedLaunch tl(e, 0, timed);
-      err = jms::launch_row_pass(e->nx, e->ny, nwave, e->waves.as<float2>(), e->n,
+      err = jms::launch_row_pass(e->nx, e->ny, nwave, (float2*)waves, e->n,
                                  e->pgr.as<float2>(), e->slcoffset[js], (long long)e->npgx * e->npgy,
                                  var, S, j, e->twx.as<float2>(), rc, mode, e->st);
       if (err != hipSuccess) return e->fail(131, "row pass", err);
@@ -299,7 +300,7 @@ int enqueue_batch(jms_hip_engine* e, int nwave, int islc0, const void* src, long
   }
   if (npass > 0 && res && ca.ndet > 0) {
     TimedLaunch tl(e, 2, timed);
-    err = jms::launch_combine(e->partial.as<float>(), ntc, e->ndet, e->ndetslc, npos, npass, weights,
+    err = jms::launch_combine(partial, ntc, e->ndet, e->ndetslc, npos, npass, weights,
                               res, accumulate, e->st);
     if (err != hipSuccess) return e->fail(133, "combine", err);
   }
@@ -793,7 +794,7 @@ int jms_hip_set_timing(jms_hip_engine* e, int enable) {
 }
 
 int jms_hip_get_timing(jms_hip_engine* e, int kind, double* ms, long long* count) {
-  if (!e || kind < 0 || kind > 2) return 1;
+  if (!e || kind < 0 || kind > 3) return 1;
   flush_timing(e);
   if (ms) *ms = e->tm[kind].ms;
   if (count) *count = e->tm[kind].count;
@@ -862,7 +863,7 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
     const float4* ppar = e->s_ppar.as<float4>() + (long long)p * npass;
     const int* var = e->s_var.as<int>() + (long long)p * npass * S;
     float* res = e->s_res.as<float>() + (long long)p * per;
-    if (e->use_graph && !e->timing) {
+    if (e->use_graph) {
       // the graph reads fixed "current batch" buffers, refreshed before replay
       hipError_t err = e->cur_ppar.ensure(sizeof(float4) * (size_t)bw);
       if (err == hipSuccess && S > 0) err = e->cur_var.ensure(sizeof(int) * (size_t)bw * S);
@@ -882,7 +883,8 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
         hipGraph_t g = nullptr;
         err = hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) return e->fail(121, "begin capture", err);
-        rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, e->cur_ppar.p, e->cur_var.as<int>(), np,
+        rc = enqueue_batch(e, e->waves.p, e->partial.as<float>(), nw, islc0, e->wav0.p, 0, e->cur_ppar.p,
+                           e->cur_var.as<int>(), np,
                            npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, BatchChans(), false);
         hipError_t err2 = hipStreamEndCapture(e->st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -893,12 +895,16 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
         if (err != hipSuccess) return e->fail(123, "instantiate graph", err);
         it = e->graphs.emplace(key, ge).first;
       }
-      err = hipGraphLaunch(it->second, e->st);
+      {
+        TimedLaunch tl(e, 3, e->timing);  // one replay = the batch's whole launch sequence
+        err = hipGraphLaunch(it->second, e->st);
+      }
       if (err == hipSuccess)
         err = hipMemcpyAsync(res, e->cur_res.p, sizeof(float) * (size_t)np * per, hipMemcpyDeviceToDevice, e->st);
       if (err != hipSuccess) return e->fail(124, "graph launch", err);
     } else {
-      rc = enqueue_batch(e, nw, islc0, e->wav0.p, 0, ppar, var, np, npass, e->s_weights.as<float>(), res,
+      rc = enqueue_batch(e, e->waves.p, e->partial.as<float>(), nw, islc0, e->wav0.p, 0, ppar, var, np, npass,
+                         e->s_weights.as<float>(), res,
                          0, BatchChans(), e->timing);
       if (rc) return rc;
     }
@@ -1011,7 +1017,8 @@ int jms_hip_multislice(jms_hip_engine* e, int islc0, const int* var, float weigh
   if (fl & JMS_HIP_DETECT_IMAGE_AVG) ch.img_avg = e->img_avg.as<float>();
   if (fl & JMS_HIP_DETECT_WAVEF_AVG) ch.wff_avg = e->wff_avg.p;
   if (fl & JMS_HIP_DETECT_WAVER_AVG) ch.wfr_avg = e->wfr_avg.p;
-  rc = enqueue_batch(e, 1, islc0, e->wav.p, e->n, nullptr, e->var1.as<int>(), 1, 1, e->w1.as<float>(),
+  rc = enqueue_batch(e, e->waves.p, e->partial.as<float>(), 1, islc0, e->wav.p, e->n, nullptr,
+                     e->var1.as<int>(), 1, 1, e->w1.as<float>(),
                      e->res1.as<float>(), 1, ch, e->timing);
   if (rc) return rc;
   // the run leaves psi(k) of the exit plane in the batch slot; keep it as the wave

@@ -8,7 +8,11 @@ Tolerances (float32 arithmetic on both sides):
     must be zero), exit waves within 1e-4 relative L2, variant sequences
     identical (drawn on the host from the same seed);
   * against the oracle's FP64 twin: GPU and float32 oracle both within 1e-4,
-    the GPU's error at most 4x the float32 oracle's or 2e-6.
+    the GPU's error at most 4x the float32 oracle's or 2e-6;
+  * a detector whose reading sits at float32's rounding-noise floor (where
+    the float32 oracle itself is more than 2.5e-5 from its FP64 twin) and
+    misses 1e-4 against the float32 oracle is judged against the FP64 twin:
+    the GPU's error there must be below 4x the float32 oracle's.
 """
 import numpy as np
 import pytest
@@ -16,7 +20,7 @@ import pytest
 from jmultislicelib_amd import Engine, JMultiSlice, configs, fft2d
 from oracle import oracle as O
 
-from helpers import assert_dets, random_setup, rel_l2
+from helpers import assert_dets, assert_dets_noise_aware, random_setup, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -269,6 +273,10 @@ def test_reference_sample_job_shape_625():
     g = eng.scan_results()
     ref = O.stem_scan(sp, dx, dy, weights, var, prec=32, nthreads=8)
     print(f"625^2 sample-job shape: {g.shape} results")
-    assert np.all(ref[:, -1, 2] > 0)  # the 80-200 mrad ring inside the band limit (~164 mrad)
-    assert_dets(g, ref, TOL, "625^2 sample-job shape")
+    # the 80-200 mrad ring lies inside the band limit (~262 mrad) but reads only
+    # ~1e-10 of the incident intensity (Gaussian potentials scatter little to
+    # high angles), at float32's noise floor
+    assert np.all(ref[:, -1, 2] > 0)
+    assert_dets_noise_aware(g, ref, lambda: O.stem_scan(sp, dx, dy, weights, var, prec=64, nthreads=8), TOL,
+                            "625^2 sample-job shape")
     eng.close()
