@@ -138,6 +138,23 @@ _SIGS = [
     ("jms_ems_write", c_int, [c_char_p, c_void_p, c_void_p]),
     ("jms_stem_images", c_int, [c_int, c_int, c_int, c_void_p, c_float, c_void_p]),
     ("jms_bin_write_channel", c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
+    # scattering factors and structure slicing (include/jms_elasca.h)
+    ("jms_weko_params", c_int, [c_int, c_void_p, c_void_p]),
+    ("jms_weko_f", c_double, [c_void_p, c_void_p, c_double]),
+    ("jms_weko_ei", c_double, [c_double]),
+    ("jms_weko_rih2", c_double, [c_double]),
+    ("jms_weko_ri1", c_double, [c_double, c_double, c_double]),
+    ("jms_weko_ri2", c_double, [c_double, c_double, c_double, c_double]),
+    ("jms_weko_imag", c_double, [c_double, c_double, c_void_p, c_void_p]),
+    ("jms_weko_form_factor", c_int, [c_double, c_double, c_int, c_double, c_int, c_int, c_void_p]),
+    ("jms_radial_form_factors", c_int, [c_int, c_float, c_float, c_int, c_int, c_float, c_longlong, c_float,
+                                        c_float, c_void_p]),
+    ("jms_form_factor_sampling", c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_void_p, c_void_p,
+                                         c_void_p]),
+    ("jms_form_factors_2d", c_int, [c_int, c_int, c_float, c_float, c_int, c_int, c_longlong, c_float, c_void_p,
+                                    c_void_p]),
+    ("jms_zslice", c_int, [c_int, c_void_p, c_float, c_float, c_void_p, c_void_p]),
+    ("jms_suggest_zslices", c_int, [c_int, c_void_p, c_void_p, c_float, c_float]),
     # specimen builder (include/jms_specimen.h)
     ("jms_perovskite_sites", c_int, [c_int, c_int, c_double, c_int, c_int, c_int, c_int, c_double, c_int,
                                      c_int, c_void_p, c_void_p]),

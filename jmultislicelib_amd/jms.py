@@ -295,9 +295,9 @@ class Engine:
         nv = _i32(sp.nvarslc)
         self._c("jms_hip_setup_gratings", sp.nx, sp.ny, len(nv), ptr(nv))
         self.sp = sp
-        if sp.gratings is None:  # generated on the device from the atom sets
+        if sp.gratings is None:  # generated on the device from the atom sets, or later by the caller
             from .configs import SIG_INT, SIGMA_POT
-            for i, sets in enumerate(sp.grating_sites):
+            for i, sets in enumerate(sp.grating_sites or []):
                 self.gaussian_gratings(i, sets, SIGMA_POT, SIG_INT)
         else:
             for i, g in enumerate(sp.gratings):

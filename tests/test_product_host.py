@@ -29,7 +29,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header", ["JMultiSliceLib.h", "jms_hip.h", "jms_specimen.h", "jms_sequence.h", "jms_io.h",
-                                    "jms_nn.h"])
+                                    "jms_nn.h", "jms_elasca.h"])
 def test_abi_exports_every_declared_symbol(header):
     import subprocess
     out = subprocess.run(["nm", "-D", "--defined-only", J.LIB_PATH], capture_output=True, text=True).stdout
