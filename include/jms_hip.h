@@ -122,6 +122,21 @@ int jms_hip_stage_scan(jms_hip_engine* e, int npos, const float* dx, const float
  * gets sum_r weights[r] * (readout of pass r) on its detection planes. */
 int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0);
 int jms_hip_sync(jms_hip_engine* e);
+/* Allocation-free batch (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops).  Runs the slice loop from plane islc0 on nwave = npos * npass waves
+ * of the set-up specimen, as one batch of jms_hip_run_staged does, with every
+ * buffer in device memory that the caller owns: waves (nwave * nx * ny
+ * complex64, working and exit waves), ppar (float4 per wave: dx, dy [nm],
+ * dz * lambda / 2, 0), var (nobjslc ints per wave), weights (npass), res
+ * (npos * ndetslc * ndet floats, overwritten).  Enqueued on the engine's
+ * stream; jms_hip_sync waits.  The readout partial sums use a fixed 16 MiB
+ * scratch array per GPU (nwave * ndetslc * ndet * column tiles floats at
+ * most 4 Mi; larger batches are refused with code 5); calls must not overlap
+ * on one GPU, since all share that GPU's scratch.  No hipMalloc, hipFree or
+ * allocating library call. */
+int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* waves, const float* ppar,
+                              const int* var, int npos, int npass, const float* weights, float* res);
 /* integrated results of the staged scan: npos*ndetslc*ndet floats,
  * [pos][idetslc][idet] */
 int jms_hip_get_scan_results(jms_hip_engine* e, float* out);

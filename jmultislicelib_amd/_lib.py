@@ -113,6 +113,8 @@ _SIGS = [
                                    c_void_p]),
     ("jms_hip_run_staged", c_int, [c_void_p, c_int, c_int, c_int]),
     ("jms_hip_sync", c_int, [c_void_p]),
+    ("jms_hip_run_batch_noalloc", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                          c_void_p, c_void_p]),
     ("jms_hip_get_scan_results", c_int, [c_void_p, c_void_p]),
     ("jms_hip_set_batch", c_int, [c_void_p, c_int]),
     ("jms_hip_set_graph", c_int, [c_void_p, c_int]),

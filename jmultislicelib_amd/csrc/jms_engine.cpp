@@ -103,6 +103,8 @@ struct jms_hip_engine {
   bool use_graph = false;
   std::map<std::array<int, 4>, hipGraphExec_t> graphs;
   std::array<const void*, 16> graph_bufs{};
+  // this device's scratch of the allocation-free batch entry (resolved at create)
+  float* scratch = nullptr;
   // timing
   bool timing = false;
   Timer tm[4];  // row pass, column pass, combine, graph replay
@@ -370,6 +372,13 @@ int jms_hip_create(jms_hip_engine** out) {
   if (err != hipSuccess) { delete e; return 101; }
   err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
   if (err != hipSuccess) { delete e; return 102; }
+  err = jms::scratch_address(e->device, &e->scratch);
+  if (err != hipSuccess) {
+    fprintf(stderr, "jms_hip error: scratch address: %s\n", hipGetErrorString(err));
+    (void)hipStreamDestroy(e->st);
+    delete e;
+    return 103;
+  }
   *out = e;
   return 0;
 }
@@ -910,6 +919,31 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
     }
   }
   return 0;
+}
+
+int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* waves, const float* ppar,
+                              const int* var, int npos, int npass, const float* weights, float* res) {
+  if (!e) return 1;
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->wav0.p) return e->fail(6, "probe not set");
+  if (nwave < 1 || nwave > 65535 || npos < 1 || npass < 1 || (long long)npos * npass != nwave)
+    return e->fail(2, "batch shape: need 1 <= nwave = npos * npass <= 65535");
+  if (!waves || !ppar || !weights || !res || (e->nobjslc > 0 && !var)) return e->fail(3, "missing buffer");
+  if (!(e->detflags & JMS_HIP_DETECT_INTEGRATED) || e->ndet < 1)
+    return e->fail(4, "no integrating detectors set up");
+  const long long need = (long long)nwave * std::max(1, e->ndetslc) * e->ndet * e->ntiles_col();
+  if (need > jms::kScratchFloats) {
+    char msg[200];
+    snprintf(msg, sizeof(msg), "batch needs %lld floats of readout scratch, more than the %lld available "
+             "(fewer waves per call)", need, (long long)jms::kScratchFloats);
+    return e->fail(5, msg);
+  }
+  if (!e->scratch) return e->fail(7, "no scratch address");
+  islc0 = std::max(0, std::min(islc0, e->nobjslc));
+  (void)hipSetDevice(e->device);
+  return enqueue_batch(e, waves, e->scratch, nwave, islc0, e->wav0.p, 0, ppar, var, npos, npass, weights, res, 0,
+                       BatchChans(), false);
 }
 
 int jms_hip_sync(jms_hip_engine* e) {

@@ -121,5 +121,9 @@ hipError_t launch_ff_gratings(float2* out, int nx, int ny, int nvar, const float
                               const float2* twy, hipStream_t st);
 // probe into out (n complex), partial: ceil(n/256) doubles of scratch
 hipError_t launch_probe(float2* out, double* partial, const ProbeArgsHost& a, hipStream_t st);
+// device address of the allocation-free entry's scratch (kScratchFloats
+// floats) on `device`, resolved once per device with that device current
+constexpr long long kScratchFloats = 4LL << 20;  // 16 MiB
+hipError_t scratch_address(int device, float** out);
 
 }  // namespace jms

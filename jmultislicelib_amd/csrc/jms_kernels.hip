@@ -2,6 +2,8 @@
 // the small non-templated kernels (readout combine, incident-wave offset,
 // accumulation).
 #include <hip/hip_runtime.h>
+#include <mutex>
+#include <vector>
 #include <cstdint>
 #include "jms_fft_device.hpp"
 #include "jms_kernels.h"
@@ -468,6 +470,35 @@ hipError_t launch_axpy_c(float2* dst, const float2* src, long long n, float w, h
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_axpy_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n, w);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free batch entry (jms_hip_run_batch_noalloc):
+// the readout partial sums of one batch.  A file-scope device array, so each
+// GPU the library runs on holds its own copy until the process ends; host
+// code must use the address scratch_address() resolves for the current
+// device, never the array's name.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+hipError_t scratch_address(int device, float** out) {
+  static std::mutex mu;
+  static std::vector<float*> addr;  // per device, resolved once
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t err = hipGetDevice(&cur);
+    if (err == hipSuccess && cur != device) err = hipSetDevice(device);
+    float* p = nullptr;
+    if (err == hipSuccess) err = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    if (err != hipSuccess) return err;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return hipSuccess;
 }
 
 }  // namespace jms
