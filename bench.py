@@ -280,7 +280,9 @@ def run_surrogate(args):
     t1 = time.perf_counter()
     log(f"training data: {len(smp.t)} positions on {len(train)} lattices in {t1 - t0:.1f} s")
     ens = S.Ensemble(nmembers=4, max_batch=256, seed=1, step=args.window_step)
-    hist = ens.train(smp, epochs=args.epochs, batch=256, lr=1e-3, log=log)
+    for net in ens.nets:
+        net.set_kernels(not args.fma)
+    hist = ens.train(smp, epochs=args.epochs, batch=256, lr=args.lr, log=log, augment=args.augment)
     t2 = time.perf_counter()
     log(f"training: {args.epochs} epochs x 4 members in {t2 - t1:.1f} s; final losses {hist[-1]}")
     # evaluation raster
@@ -294,14 +296,23 @@ def run_surrogate(args):
     t3 = time.perf_counter()
     plain = S.plain_scan(eng, px, py)
     t4 = time.perf_counter()
-    ws = S.Samples([test_lat], np.zeros(len(px), np.int32), px, py, plain).windows(step=ens.step)
-    net_only, spread = ens.predict(ws)
+    if args.sym:
+        x8, y8 = S.symmetric_images(px, py, test_lat.a)
+        ws = S.Samples([test_lat], np.zeros(8 * len(px), np.int32), x8.ravel(), y8.ravel(),
+                       np.tile(plain, (8, 1))).windows(step=ens.step)
+        net_only, spread = ens.predict(ws, 8)
+    else:
+        ws = S.Samples([test_lat], np.zeros(len(px), np.int32), px, py, plain).windows(step=ens.step)
+        net_only, spread = ens.predict(ws)
     ws.close()
     eng.set_batch(2048)
-    out, idx, tm = S.surrogate_scan(ens, eng, test_lat, px, py, audit=0.05, spread_max=0.02, seed=9)
+    out, idx, tm = S.surrogate_scan(ens, eng, test_lat, px, py, audit=0.05, spread_max=0.02, seed=9, sym=args.sym,
+                                    calib=args.calib)
     t_plain, t_surr = t4 - t3, tm["total_s"]
+    raw = tm.pop("uncalibrated")
     rel = lambda p, d: float(np.mean(np.abs(p[:, d] - plain[:, d]) / np.abs(plain[:, d])))
-    errs = {S.DET_NAMES[d]: {"surrogate_scan": round(rel(out, d), 5), "network_only": round(rel(net_only, d), 5)}
+    errs = {S.DET_NAMES[d]: {"surrogate_scan": round(rel(out, d), 5), "network_only": round(rel(net_only, d), 5),
+                             "fallback_without_calibration": round(rel(raw, d), 5)}
             for d in range(3)}
     log(f"plain {t_plain:.2f} s, surrogate {t_surr:.2f} s ({tm}); re-simulated {len(idx)} of {len(px)}; "
         f"errors {errs}")
@@ -318,6 +329,9 @@ def run_surrogate(args):
                            "window": {"pixels": [S.WIN, S.WIN], "step_ang": ens.step,
                                       "configured": abs(ens.step - 0.1) < 1e-9},
                            "errors": errs, "train_data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
+                           "epochs": args.epochs, "augment_4mm": bool(args.augment), "predict_4mm": bool(args.sym),
+                           "calibration_kmax": args.calib,
+                           "conv_kernels": "fma" if args.fma else "mfma",
                            "timings": {k: round(v, 3) for k, v in tm.items()},
                            "test_lattice": {"a": test_lat.a, "za": test_lat.za, "zb": test_lat.zb}},
                 "roofline": None, "cpu_baseline": None}
@@ -480,6 +494,15 @@ def main():
     ap.add_argument("--train-positions", type=int, default=400, help="surrogate: positions per lattice")
     ap.add_argument("--epochs", type=int, default=30, help="surrogate: training epochs")
     ap.add_argument("--scan", type=int, default=256, help="surrogate: evaluation scan is scan x scan")
+    ap.add_argument("--lr", type=float, default=1e-3, help="surrogate: initial learning rate")
+    ap.add_argument("--augment", action="store_true",
+                    help="surrogate: train on the 8 symmetry (4mm) images of every training position")
+    ap.add_argument("--sym", action="store_true",
+                    help="surrogate: average each prediction over the 8 symmetry images of the position")
+    ap.add_argument("--fma", action="store_true", help="surrogate: FMA convolution kernels instead of MFMA")
+    ap.add_argument("--calib", type=int, default=0, metavar="K",
+                    help="surrogate: calibrate the predictions with the re-simulated positions (periodic 4mm "
+                         "basis up to order K; 0 = off)")
     ap.add_argument("--window-step", type=float, default=0.1,
                     help="surrogate: window pixel spacing [A] (0.1 A: 32 x 32 pixels over 3.2 A, the configured input)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -39,6 +39,11 @@ typedef struct jms_nn_data jms_nn_data;
  * (He-normal, zero biases) from seed */
 int jms_nn_create(jms_nn** out, int max_batch, unsigned int seed);
 void jms_nn_destroy(jms_nn* m);
+/* kernels of the convolutions: mfma != 0 (the default) runs them as implicit
+ * GEMMs on the matrix cores (bf16 MFMA, FP32 accumulation; the output
+ * gradient is rounded to bf16 for the weight and data gradients), 0 on the
+ * FMA kernels (FP32 output gradient) */
+int jms_nn_set_kernels(jms_nn* m, int mfma);
 int jms_nn_num_params(void);
 int jms_nn_get_params(jms_nn* m, float* out);
 int jms_nn_set_params(jms_nn* m, const float* in);

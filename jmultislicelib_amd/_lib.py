@@ -164,6 +164,7 @@ _SIGS = [
                                            c_double, c_double, c_void_p]),
     # surrogate network (include/jms_nn.h)
     ("jms_nn_create", c_int, [c_void_p, c_int, c_uint]),
+    ("jms_nn_set_kernels", c_int, [c_void_p, c_int]),
     ("jms_nn_destroy", None, [c_void_p]),
     ("jms_nn_num_params", c_int, []),
     ("jms_nn_get_params", c_int, [c_void_p, c_void_p]),

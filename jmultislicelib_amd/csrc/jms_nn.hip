@@ -41,6 +41,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
+}  // namespace
+
+#include "jms_nn_mfma.hpp"
+
+namespace {
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -186,16 +192,25 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ dy
 }
 
 // y[n][o] = b[o] + sum_i x[n][i] w[o][i]; relu -> bf16 out, else fp32 out
+// (nin a multiple of 8: eight bf16 per 16-byte load)
 __global__ void k_dense_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
                             const float* __restrict__ b, int n, int nin, int nout, uint16_t* __restrict__ ybf,
                             float* __restrict__ yf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * nout) return;
   const int o = i % nout, s = i / nout;
-  const uint16_t* xs = x + (long long)s * nin;
-  const uint16_t* wo = w + (long long)o * nin;
+  const uint4* xs = reinterpret_cast<const uint4*>(x + (long long)s * nin);
+  const uint4* wo = reinterpret_cast<const uint4*>(w + (long long)o * nin);
   float acc = b[o];
-  for (int k = 0; k < nin; ++k) acc = fmaf(bf2f(xs[k]), bf2f(wo[k]), acc);
+  for (int k = 0; k < nin / 8; ++k) {
+    const uint4 xv = xs[k], wv = wo[k];
+    const uint32_t xa[4] = {xv.x, xv.y, xv.z, xv.w}, wa[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc = fmaf(__uint_as_float(xa[j] << 16), __uint_as_float(wa[j] << 16), acc);
+      acc = fmaf(__uint_as_float(xa[j] & 0xFFFF0000u), __uint_as_float(wa[j] & 0xFFFF0000u), acc);
+    }
+  }
   if (ybf) ybf[i] = f2bf(fmaxf(acc, 0.f));
   else yf[i] = acc;
 }
@@ -215,19 +230,22 @@ __global__ void k_dense_bwd_data(const float* __restrict__ dy, const uint16_t* _
   dx[i] = acc;
 }
 
-// dw[o][i] = sum_n dy[n][o] x[n][i]; threads i == nin write db[o] = sum_n dy[n][o]
+// dw[o][i] += sum over this block's share of the samples of dy[n][o] x[n][i];
+// threads i == nin add db[o] += sum dy[n][o]; dw and db are cleared first.
+// grid: (ceil(nout (nin + 1) / 256), shares)
 __global__ void k_dense_bwd_w(const float* __restrict__ dy, const uint16_t* __restrict__ x, int n, int nin,
                               int nout, float* __restrict__ dw, float* __restrict__ db) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nout * (nin + 1)) return;
   const int o = i / (nin + 1), k = i % (nin + 1);
+  const int s0 = (int)((long long)n * blockIdx.y / gridDim.y), s1 = (int)((long long)n * (blockIdx.y + 1) / gridDim.y);
   float acc = 0.f;
   if (k < nin) {
-    for (int s = 0; s < n; ++s) acc = fmaf(dy[s * nout + o], bf2f(x[(long long)s * nin + k]), acc);
-    dw[(long long)o * nin + k] = acc;
+    for (int s = s0; s < s1; ++s) acc = fmaf(dy[s * nout + o], bf2f(x[(long long)s * nin + k]), acc);
+    atomicAdd(&dw[(long long)o * nin + k], acc);
   } else {
-    for (int s = 0; s < n; ++s) acc += dy[s * nout + o];
-    db[o] = acc;
+    for (int s = s0; s < s1; ++s) acc += dy[s * nout + o];
+    atomicAdd(&db[o], acc);
   }
 }
 
@@ -351,6 +369,8 @@ struct jms_nn {
   int perm_cap = 0;
   float* eloss = nullptr;    // per-batch losses of an epoch (device), capacity eloss_cap
   int eloss_cap = 0;
+  float* part = nullptr;     // per-sample weight-gradient partials of the matrix-core path
+  bool mfma = true;          // convolutions on the matrix cores (else the FMA kernels)
   long long step = 0;
 };
 
@@ -377,14 +397,61 @@ void release(jms_nn* m) {
   for (auto* a : m->act) (void)hipFree(a);
   for (auto* a : m->gbuf) (void)hipFree(a);
   (void)hipFree(m->out); (void)hipFree(m->tb); (void)hipFree(m->scale); (void)hipFree(m->loss_part);
-  (void)hipFree(m->idx); (void)hipFree(m->perm); (void)hipFree(m->eloss);
+  (void)hipFree(m->idx); (void)hipFree(m->perm); (void)hipFree(m->eloss); (void)hipFree(m->part);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
+}
+
+// matrix-core convolution of layer l (one block per sample)
+void conv_fwd_mfma(jms_nn* m, int l, int n) {
+  const dim3 g(n), b(256);
+  const uint16_t* x = m->act[l];
+  const uint16_t* w = m->pbf + off_cw(l);
+  const float* bias = m->p + off_cb(l);
+  uint16_t* y = m->act[l + 1];
+  switch (l) {
+    case 0: hipLaunchKernelGGL((nnmfma::k_conv_fwd<1, C0, H0>), g, b, 0, m->st, x, w, bias, y); break;
+    case 1: hipLaunchKernelGGL((nnmfma::k_conv_fwd<2, CH, H0>), g, b, 0, m->st, x, w, bias, y); break;
+    case 2: hipLaunchKernelGGL((nnmfma::k_conv_fwd<2, CH, H0 / 2>), g, b, 0, m->st, x, w, bias, y); break;
+    default: hipLaunchKernelGGL((nnmfma::k_conv_fwd<2, CH, H0 / 4>), g, b, 0, m->st, x, w, bias, y); break;
+  }
+}
+
+// matrix-core gradients of layer l: weights (per-sample partials reduced in
+// a fixed order), bias, and for l > 0 the gradient w.r.t. its input
+void conv_bwd_mfma(jms_nn* m, int l, int n) {
+  const dim3 g(n), b(256);
+  const float* dy = m->gbuf[l];
+  const uint16_t* x = m->act[l];
+  switch (l) {
+    case 0: hipLaunchKernelGGL((nnmfma::k_conv_bwd_w<1, C0, H0>), g, b, 0, m->st, dy, x, m->part); break;
+    case 1: hipLaunchKernelGGL((nnmfma::k_conv_bwd_w<2, CH, H0>), g, b, 0, m->st, dy, x, m->part); break;
+    case 2: hipLaunchKernelGGL((nnmfma::k_conv_bwd_w<2, CH, H0 / 2>), g, b, 0, m->st, dy, x, m->part); break;
+    default: hipLaunchKernelGGL((nnmfma::k_conv_bwd_w<2, CH, H0 / 4>), g, b, 0, m->st, dy, x, m->part); break;
+  }
+  const int cnt = conv_w(l);
+  const unsigned shares = (unsigned)std::max(1, std::min(n / 8, 32));
+  hipLaunchKernelGGL(nnmfma::k_reduce_parts, dim3(nblk(cnt, 256), shares), dim3(256), 0, m->st, m->part, n, cnt,
+                     m->g + off_cw(l));
+  hipLaunchKernelGGL(nnmfma::k_bias_grad, dim3(CH, shares), dim3(256), 0, m->st, dy, n, HOUT[l] * HOUT[l],
+                     m->g + off_cb(l));
+  if (l == 0) return;
+  const uint16_t* w = m->pbf + off_cw(l);
+  float* dx = m->gbuf[l - 1];
+  switch (l) {
+    case 1: hipLaunchKernelGGL((nnmfma::k_conv_bwd_data<2, H0>), g, b, 0, m->st, dy, w, x, dx); break;
+    case 2: hipLaunchKernelGGL((nnmfma::k_conv_bwd_data<2, H0 / 2>), g, b, 0, m->st, dy, w, x, dx); break;
+    default: hipLaunchKernelGGL((nnmfma::k_conv_bwd_data<2, H0 / 4>), g, b, 0, m->st, dy, w, x, dx); break;
+  }
 }
 
 // forward over act[0] (n samples) -> out
 hipError_t forward(jms_nn* m, int n) {
   for (int l = 0; l < NCONV; ++l) {
+    if (m->mfma) {
+      conv_fwd_mfma(m, l, n);
+      continue;
+    }
     const long long px = (long long)n * HOUT[l] * HOUT[l];
     const int gsz = group_size(px, CH);
     const dim3 g(nblk(px, 256), CH / gsz);
@@ -413,21 +480,27 @@ hipError_t backward(jms_nn* m, int n, float* loss_acc = nullptr) {
   const unsigned nb = nblk((long long)n * NOUT, 256);
   float* gout = m->gbuf[NCONV + 1];
   float* gh = m->gbuf[NCONV];
+  // every gradient is accumulated over shares of the batch: clear them all
+  hipLaunchKernelGGL(k_zero, dim3(nblk(NPARAM, 256)), dim3(256), 0, m->st, m->g, NPARAM);
   hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, m->st, m->out, m->tb, m->scale, n, gout, m->loss_part,
                      loss_acc);
+  const unsigned shares = (unsigned)std::max(1, std::min(n / 16, 16));
   // dense 2
-  hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((long long)NOUT * (HID + 1), 256)), dim3(256), 0, m->st, gout,
-                     m->act[NCONV + 1], n, HID, NOUT, m->g + OFF_D2W, m->g + OFF_D2B);
+  hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((long long)NOUT * (HID + 1), 256), shares), dim3(256), 0, m->st,
+                     gout, m->act[NCONV + 1], n, HID, NOUT, m->g + OFF_D2W, m->g + OFF_D2B);
   hipLaunchKernelGGL(k_dense_bwd_data, dim3(nblk((long long)n * HID, 256)), dim3(256), 0, m->st, gout,
                      m->pbf + OFF_D2W, m->act[NCONV + 1], n, HID, NOUT, gh);
   // dense 1
-  hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((long long)HID * (FLAT + 1), 256)), dim3(256), 0, m->st, gh,
-                     m->act[NCONV], n, FLAT, HID, m->g + OFF_D1W, m->g + OFF_D1B);
+  hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((long long)HID * (FLAT + 1), 256), shares), dim3(256), 0, m->st,
+                     gh, m->act[NCONV], n, FLAT, HID, m->g + OFF_D1W, m->g + OFF_D1B);
   hipLaunchKernelGGL(k_dense_bwd_data, dim3(nblk((long long)n * FLAT, 256)), dim3(256), 0, m->st, gh,
                      m->pbf + OFF_D1W, m->act[NCONV], n, FLAT, HID, m->gbuf[NCONV - 1]);
   // convolutions, last to first; gbuf[l] = gradient w.r.t. the output of conv l+1;
   // their weight gradients are accumulated atomically over shares of the batch
-  hipLaunchKernelGGL(k_zero, dim3(nblk(OFF_D1W, 256)), dim3(256), 0, m->st, m->g, OFF_D1W);
+  if (m->mfma) {
+    for (int l = NCONV - 1; l >= 0; --l) conv_bwd_mfma(m, l, n);
+    return hipGetLastError();
+  }
   for (int l = NCONV - 1; l >= 0; --l) {
     const float* dy = m->gbuf[l];
     const int nblk_w = CIN[l] * (CH / GMAX);
@@ -493,6 +566,7 @@ int jms_nn_create(jms_nn** out, int max_batch, unsigned int seed) {
   if (e == hipSuccess) e = dalloc(&m->scale, NOUT);
   if (e == hipSuccess) e = dalloc(&m->loss_part, nblk((long long)max_batch * NOUT, 256));
   if (e == hipSuccess) e = dalloc(&m->idx, max_batch);
+  if (e == hipSuccess) e = dalloc(&m->part, (size_t)max_batch * conv_w(1));  // the largest layer
   if (e != hipSuccess) {
     release(m);
     return 2;
@@ -512,6 +586,12 @@ int jms_nn_create(jms_nn** out, int max_batch, unsigned int seed) {
 }
 
 void jms_nn_destroy(jms_nn* m) { release(m); }
+
+int jms_nn_set_kernels(jms_nn* m, int mfma) {
+  if (!m) return 1;
+  m->mfma = mfma != 0;
+  return 0;
+}
 
 int jms_nn_set_params(jms_nn* m, const float* in) {
   if (!m || !in) return 1;

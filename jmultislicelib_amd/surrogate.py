@@ -67,6 +67,10 @@ class Network:
         except Exception:
             pass
 
+    def set_kernels(self, mfma: bool):
+        """convolutions on the matrix cores (default) or the FMA kernels"""
+        _chk("jms_nn_set_kernels", lib().jms_nn_set_kernels(self.h, int(mfma)))
+
     @staticmethod
     def num_params() -> int:
         return lib().jms_nn_num_params()
@@ -250,6 +254,23 @@ def plain_scan(eng: Engine, px_ang, py_ang) -> np.ndarray:
     return eng.scan_results()[:, -1, :]
 
 
+# the point group 4mm of the perovskite-like cells about the A column at the
+# origin: (x, y) -> (m00 x + m01 y, m10 x + m11 y).  Each slice type, and so
+# the whole stack, maps onto itself, and so does the round aberration-free
+# probe: the detector intensities at the 8 images of a position are equal.
+SYM_4MM = ((1, 0, 0, 1), (0, -1, 1, 0), (-1, 0, 0, -1), (0, 1, -1, 0),
+           (1, 0, 0, -1), (-1, 0, 0, 1), (0, 1, 1, 0), (0, -1, -1, 0))
+
+
+def symmetric_images(px, py, a):
+    """The 8 images [8][n] of positions (px, py) [A] under 4mm about the
+    origin, wrapped into the cell [0, a)."""
+    px, py = np.asarray(px, np.float64), np.asarray(py, np.float64)
+    xs = np.stack([np.mod(m00 * px + m01 * py, a) for m00, m01, _, _ in SYM_4MM])
+    ys = np.stack([np.mod(m10 * px + m11 * py, a) for _, _, m10, m11 in SYM_4MM])
+    return xs.astype(np.float32), ys.astype(np.float32)
+
+
 @dataclass
 class Samples:
     """Positions on several crystals with their multislice intensities."""
@@ -258,6 +279,20 @@ class Samples:
     px: np.ndarray      # [A]
     py: np.ndarray
     t: np.ndarray       # [n][3]
+
+    def augmented(self) -> "Samples":
+        """The samples at all 8 symmetry images of their positions, with the
+        same intensities (8x the training data without new simulations)."""
+        xs, ys = [], []
+        for k, lat in enumerate(self.lattices):
+            sel = self.cid == k
+            x8, y8 = symmetric_images(self.px[sel], self.py[sel], lat.a)
+            xs.append((x8, y8, sel))
+        px = np.zeros((8, len(self.px)), np.float32)
+        py = np.zeros((8, len(self.py)), np.float32)
+        for x8, y8, sel in xs:
+            px[:, sel], py[:, sel] = x8, y8
+        return Samples(self.lattices, np.tile(self.cid, 8), px.ravel(), py.ravel(), np.tile(self.t, (8, 1)))
 
     def windows(self, nslices: int = 200, scale=None, step: float = STEP_ANG) -> WindowSet:
         """device windows (pixel spacing step [A]) with targets t, or
@@ -299,12 +334,15 @@ class Ensemble:
         self.step = float(step)
 
     def train(self, samples: Samples, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0,
-              nslices: int = 200, log=None):
+              nslices: int = 200, log=None, augment: bool = False):
         """Adam on the mean squared error of log(intensity / detector mean):
         the three detectors differ by orders of magnitude and the loss then
         measures relative errors; the learning rate decays linearly to lr/10.
+        augment: train on the 8 symmetry images of every position.
         Returns the per-epoch losses [epochs][members]."""
         self.scale = np.asarray(samples.t, np.float64).mean(axis=0).astype(np.float32)
+        if augment:
+            samples = samples.augmented()
         data = samples.windows(nslices, self.scale, self.step)
         one = np.ones(NOUT, np.float32)
         hist = []
@@ -317,10 +355,15 @@ class Ensemble:
         data.close()
         return np.array(hist)
 
-    def predict(self, data: WindowSet):
+    def predict(self, data: WindowSet, nsym: int = 1):
         """(mean [n][3], relative spread [n][3] = std / |mean|) in intensity
-        units"""
-        p = np.exp(np.stack([net.predict(data) for net in self.nets]).astype(np.float64)) * self.scale
+        units.  nsym = 8: data holds the 8 symmetry images of n positions
+        (image-major); each member's log-predictions are averaged over the
+        images before the ensemble statistics."""
+        lp = np.stack([net.predict(data) for net in self.nets]).astype(np.float64)
+        if nsym > 1:
+            lp = lp.reshape(len(self.nets), nsym, -1, NOUT).mean(axis=1)
+        p = np.exp(lp) * self.scale
         mean = p.mean(axis=0)
         spread = p.std(axis=0) / np.maximum(np.abs(mean), 1e-30)
         return mean, spread
@@ -330,19 +373,55 @@ class Ensemble:
             n.close()
 
 
+def _cell_basis(u, v, kmax: int) -> np.ndarray:
+    """Periodic functions of the fractional position (u, v) in the cell that
+    are invariant under 4mm about the origin: cos(2 pi h u) cos(2 pi k v) +
+    cos(2 pi k u) cos(2 pi h v), 0 <= h <= k <= kmax.  [n][nbasis]"""
+    u, v = np.asarray(u, np.float64), np.asarray(v, np.float64)
+    cols = []
+    for k in range(kmax + 1):
+        for h in range(k + 1):
+            cols.append(np.cos(2 * np.pi * h * u) * np.cos(2 * np.pi * k * v) +
+                        np.cos(2 * np.pi * k * u) * np.cos(2 * np.pi * h * v))
+    return np.stack(cols, axis=1)
+
+
+def calibrate(pred, truth, u_fit, v_fit, u_all, v_all, kmax: int = 6, ridge: float = 1e-6):
+    """Correction factors [n_all][3] for predictions at (u_all, v_all) from
+    the re-simulated positions (u_fit, v_fit) where both the prediction and
+    the multislice value are known: log(truth / pred) is fitted per detector
+    by least squares (small ridge) on the 4mm-symmetric periodic basis of the
+    cell, which is where a surrogate's error on one crystal varies (the
+    network's error is a smooth function of the probe position in the cell)."""
+    A = _cell_basis(u_fit, v_fit, kmax)
+    r = np.log(np.maximum(truth, 1e-30) / np.maximum(pred, 1e-30))
+    G = A.T @ A + ridge * len(A) * np.eye(A.shape[1])
+    coef = np.linalg.solve(G, A.T @ r)
+    return np.exp(_cell_basis(u_all, v_all, kmax) @ coef)
+
+
 def surrogate_scan(ens: Ensemble, eng: Engine, lat: Lattice, px_ang, py_ang, audit: float = 0.05,
-                   spread_max: float = 0.02, seed: int = 0, nslices: int = 200, dets=(1, 2)):
+                   spread_max: float = 0.02, seed: int = 0, nslices: int = 200, dets=(1, 2), sym: bool = False,
+                   calib: int = 0):
     """Intensities [npos][3] of a scan on lat (engine set up for it): the
-    ensemble predicts every position; a random fraction `audit` and every
-    position whose ensemble spread exceeds spread_max on the detectors
-    `dets` are re-simulated by the plain multislice, which replaces the
-    prediction there.  Returns (intensities, indices re-simulated, timings)."""
+    ensemble predicts every position (sym: averaged over the 8 symmetry
+    images of the position); a random fraction `audit` and every position
+    whose ensemble spread exceeds spread_max on the detectors `dets` are
+    re-simulated by the plain multislice, which replaces the prediction
+    there.  calib > 0: the re-simulated positions also calibrate the other
+    predictions (`calibrate` with kmax = calib).  Returns (intensities,
+    indices re-simulated, timings)."""
     t0 = time.perf_counter()
     n = len(px_ang)
-    ws = Samples([lat], np.zeros(n, np.int32), np.asarray(px_ang, np.float32), np.asarray(py_ang, np.float32),
-                 np.zeros((n, NOUT), np.float32)).windows(nslices, step=ens.step)
+    px, py = np.asarray(px_ang, np.float32), np.asarray(py_ang, np.float32)
+    nsym = 8 if sym else 1
+    if sym:
+        x8, y8 = symmetric_images(px, py, lat.a)
+        px, py = x8.ravel(), y8.ravel()
+    ws = Samples([lat], np.zeros(n * nsym, np.int32), px, py,
+                 np.zeros((n * nsym, NOUT), np.float32)).windows(nslices, step=ens.step)
     t1 = time.perf_counter()
-    mean, spread = ens.predict(ws)
+    mean, spread = ens.predict(ws, nsym)
     ws.close()
     t2 = time.perf_counter()
     rng = np.random.default_rng(seed)
@@ -354,4 +433,13 @@ def surrogate_scan(ens: Ensemble, eng: Engine, lat: Lattice, px_ang, py_ang, aud
         eng.set_batch(len(idx))
         out[idx] = plain_scan(eng, np.asarray(px_ang)[idx], np.asarray(py_ang)[idx])
     t3 = time.perf_counter()
-    return out, idx, {"windows_s": t1 - t0, "inference_s": t2 - t1, "fallback_s": t3 - t2, "total_s": t3 - t0}
+    raw = out.copy()
+    if calib > 0 and len(idx) > 0:
+        u = np.mod(np.asarray(px_ang, np.float64) / lat.a, 1.0)
+        v = np.mod(np.asarray(py_ang, np.float64) / lat.a, 1.0)
+        rest = np.setdiff1d(np.arange(n), idx)
+        fac = calibrate(mean[idx], out[idx], u[idx], v[idx], u[rest], v[rest], kmax=calib)
+        out[rest] = mean[rest] * fac
+    t4 = time.perf_counter()
+    return out, idx, {"windows_s": t1 - t0, "inference_s": t2 - t1, "fallback_s": t3 - t2,
+                      "calibration_s": t4 - t3, "total_s": t4 - t0, "uncalibrated": raw}

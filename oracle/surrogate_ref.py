@@ -5,7 +5,10 @@ its forward pass and gradients are checked against this float64 evaluation.
 Test infrastructure: only tests/ may use it.  With quantize=True the values
 are rounded to bf16 at the points where the device stores bf16 (input,
 weights' compute copies, every ReLU activation); with quantize=False the
-network is evaluated in plain float64, which finite differences pin."""
+network is evaluated in plain float64, which finite differences pin.  With
+round_dy=True the gradient at each convolution's output is also rounded to
+bf16 before it meets the weights and the layer input, as the matrix-core
+kernels take it (the bias gradient sums the unrounded values)."""
 from __future__ import annotations
 
 import numpy as np
@@ -96,7 +99,7 @@ def forward(p, x, quantize=True):
     return y, acts
 
 
-def loss_and_grads(p, x, t, scale, quantize=True):
+def loss_and_grads(p, x, t, scale, quantize=True, round_dy=False):
     q = bf16 if quantize else (lambda a: np.asarray(a, np.float64))
     P = unpack(p)
     y, acts = forward(p, x, quantize)
@@ -115,8 +118,8 @@ def loss_and_grads(p, x, t, scale, quantize=True):
     G["dense1.b"] = dh.sum(0)
     da = ((dh @ q(P["dense1.w"])) * (f > 0)).reshape(acts[4].shape)
     for l in range(3, -1, -1):
-        dw, db, dx = _conv_back(acts[l], q(P[f"conv{l + 1}.w"]), da, STRIDES[l])
-        G[f"conv{l + 1}.w"], G[f"conv{l + 1}.b"] = dw, db
+        dw, _, dx = _conv_back(acts[l], q(P[f"conv{l + 1}.w"]), bf16(da) if round_dy else da, STRIDES[l])
+        G[f"conv{l + 1}.w"], G[f"conv{l + 1}.b"] = dw, da.sum(axis=(0, 2, 3))
         if l > 0:
             da = dx * (acts[l] > 0)
     return loss, pack(G)

@@ -20,9 +20,14 @@ from helpers import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def test_forward_and_gradients_vs_cpu():
+@pytest.mark.parametrize("mfma", [True, False])
+def test_forward_and_gradients_vs_cpu(mfma):
+    """Matrix-core convolutions (bf16 MFMA, the output gradient rounded to
+    bf16) and the FMA kernels, each against the restatement with the same
+    rounding points."""
     rng = np.random.default_rng(4)
     net = S.Network(max_batch=8, seed=3)
+    net.set_kernels(mfma)
     p = net.params().astype(np.float64)
     x = (rng.standard_normal((6, 8, 32, 32)) * 0.5).astype(np.float32)
     xb = R.bf16_bits(x)
@@ -32,7 +37,7 @@ def test_forward_and_gradients_vs_cpu():
     yr, _ = R.forward(p, R.bf16(x))
     ey = rel_l2(y, yr)
     loss, g = net.gradients(xb, t, sc)
-    lr_, gr = R.loss_and_grads(p, R.bf16(x), t, sc)
+    lr_, gr = R.loss_and_grads(p, R.bf16(x), t, sc, round_dy=mfma)
     print(f"forward rel L2 {ey:.2e}; loss {loss:.6e} vs {lr_:.6e}")
     assert ey < 2e-3 and abs(loss - lr_) <= 2e-3 * abs(lr_)
     o = 0
@@ -110,3 +115,22 @@ def test_training_gain_and_audited_scan():
     ens.close()
     untrained.close()
     eng.close()
+
+
+def test_symmetry_images_give_equal_intensities():
+    """The perovskite-like stack, the round probe and the annular detectors
+    are invariant under 4mm about the A column at the origin, so the plain
+    multislice gives the same intensities at the 8 images of a position (the
+    premise of training on the images and averaging predictions over them)."""
+    lat = S.random_lattice(np.random.default_rng(31))
+    eng = Engine()
+    eng.setup(S.lattice_setup(lat, n=256, ncell=4))
+    x8, y8 = S.symmetric_images([0.83, 1.37, 0.2], [0.29, 0.61, 1.9], lat.a)
+    eng.set_batch(24)
+    t = S.plain_scan(eng, x8.ravel(), y8.ravel()).reshape(8, 3, 3).astype(np.float64)
+    eng.close()
+    spread = np.abs(t - t[0]).max(axis=0) / np.abs(t[0])
+    print("max relative deviation over the images [position][detector]:\n", spread)
+    assert spread.max() < 2e-4
+    # and different positions do differ
+    assert np.abs(t[0, 0] - t[0, 1]).max() > 1e-2 * np.abs(t[0, 0]).max()
