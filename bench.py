@@ -200,10 +200,13 @@ def run_jobs64(args):
     """Uneven batch: 64 random orthorhombic crystal jobs (seed 2), grid edge
     from {360, 384, 480, 512, 640, 720} at 0.06 A, 50-400 slices of c/2,
     16-256 probe positions each (one pass), 20-30 mrad apertures, detectors
-    0-11, 11-22, 70-200 mrad.  Every job has its own engine and HIP stream;
-    all jobs are enqueued before the first wait, so jobs of different sizes
-    run at the same time.  One step = the whole batch."""
-    from jmultislicelib_amd import Engine, configs
+    0-11, 11-22, 70-200 mrad.  Every job has its own engine.  By default the
+    jobs of each grid edge form a group (jms_hip_group): one row and one
+    column launch per slice step over the waves of all its jobs, the six
+    groups on six streams.  --streams: each job's own launch sequence on its
+    own stream, all enqueued before the first wait.  One step = the whole
+    batch."""
+    from jmultislicelib_amd import Engine, Group, configs
     t0 = time.perf_counter()
     jobs = configs.uneven_jobs(64, seed=2)
     engines = []
@@ -216,23 +219,32 @@ def run_jobs64(args):
         engines.append(eng)
     for eng in engines:
         eng.sync()
+    grouped = not args.streams
+    groups = []
+    if grouped:  # one group per grid edge: one launch per pass and step over all its jobs
+        for e in sorted(set(j.setup.nx for j in jobs)):
+            groups.append(Group([eng for eng, j in zip(engines, jobs) if j.setup.nx == e]))
     positions = sum(len(j.dx) for j in jobs)
     work = sum(len(j.dx) * len(j.setup.objslc) * 32.0 * j.setup.nx * j.setup.ny for j in jobs)
     log(f"setup {time.perf_counter() - t0:.1f} s: {len(jobs)} jobs, {positions} positions, "
         f"edges {sorted(set(j.setup.nx for j in jobs))}, slices {min(len(j.setup.objslc) for j in jobs)}-"
         f"{max(len(j.setup.objslc) for j in jobs)}")
-    for _ in range(args.warmup):  # the last 10 slices of every job
-        for eng, j in zip(engines, jobs):
-            eng.run_staged(0, len(j.dx), len(j.setup.objslc) - 10)
+    def step():
+        if grouped:
+            for g in groups:  # each group on its first member's stream: the edges run concurrently
+                g.run()
+        else:
+            for eng, j in zip(engines, jobs):
+                eng.run_staged(0, len(j.dx))
         for eng in engines:
             eng.sync()
+
+    for _ in range(args.warmup):
+        step()
     times = []
     for _ in range(args.steps):
         t1 = time.perf_counter()
-        for eng, j in zip(engines, jobs):
-            eng.run_staged(0, len(j.dx))
-        for eng in engines:
-            eng.sync()
+        step()
         times.append(time.perf_counter() - t1)
     wall = sum(times)
     finite = all(np.isfinite(eng.scan_results()).all() for eng in engines)
@@ -245,13 +257,17 @@ def run_jobs64(args):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 1),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
-           "config": {"workload": "jobs64: 64 random orthorhombic crystal jobs, seed 2, one engine and "
-                                  "stream per job, all enqueued before the first wait",
-                      "jobs": len(jobs), "positions_per_step": positions},
+           "config": {"workload": "jobs64: 64 random orthorhombic crystal jobs, seed 2, " +
+                                  ("grouped launches per grid edge (one row and one column launch per slice "
+                                   "step over all jobs of the edge), the six edges on six streams" if grouped else
+                                   "one engine and stream per job, all enqueued before the first wait"),
+                      "jobs": len(jobs), "positions_per_step": positions, "grouped": grouped},
            "roofline": {"bound": "hbm", "achieved": round(rate, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(rate / HBM_PEAK_GBS, 4), "traffic": None,
                         "kernel": "whole slice loop over all jobs (32 n bytes per position and slice)"},
            "cpu_baseline": None}
+    for g in groups:
+        g.close()
     for eng in engines:
         eng.close()
     print(json.dumps(out), flush=True)
@@ -263,12 +279,16 @@ def run_surrogate(args):
     B-site Z in [22, 74], O), 512^2 over 8x8 cells, 200 slices, 25 mrad probe,
     `--train-positions` random positions per lattice, one multislice pass per
     position (no frozen phonons, so the plain reference is deterministic).
-    An ensemble of 4 networks is trained by the library's kernels.  The
+    An ensemble of 4 networks is trained by the library's kernels, by
+    default on the 8 images of every training position under the lattices'
+    point group 4mm (the intensities are equal there), and by default each
+    prediction is averaged over the 8 images of its position.  The
     evaluation is a scan x scan raster over one cell of an unseen lattice:
     the plain batched multislice of every position against the surrogate
     scan (device windows, ensemble inference, 5% random audit plus every
     position whose ensemble spread exceeds 2% on ABF or HAADF re-simulated by
-    the plain multislice)."""
+    the plain multislice; with --calib K the re-simulated positions also
+    calibrate the other predictions, inside the timed scan)."""
     from jmultislicelib_amd import Engine
     from jmultislicelib_amd import surrogate as S
     rng = np.random.default_rng(2024)
@@ -279,7 +299,7 @@ def run_surrogate(args):
     smp = S.simulate(train, args.train_positions, seed=5, eng=eng)
     t1 = time.perf_counter()
     log(f"training data: {len(smp.t)} positions on {len(train)} lattices in {t1 - t0:.1f} s")
-    ens = S.Ensemble(nmembers=4, max_batch=256, seed=1, step=args.window_step)
+    ens = S.Ensemble(nmembers=4, max_batch=2048, seed=1, step=args.window_step)  # batches of 2048 at inference
     for net in ens.nets:
         net.set_kernels(not args.fma)
     hist = ens.train(smp, epochs=args.epochs, batch=256, lr=args.lr, log=log, augment=args.augment)
@@ -311,8 +331,11 @@ def run_surrogate(args):
     t_plain, t_surr = t4 - t3, tm["total_s"]
     raw = tm.pop("uncalibrated")
     rel = lambda p, d: float(np.mean(np.abs(p[:, d] - plain[:, d]) / np.abs(plain[:, d])))
+    pred = np.setdiff1d(np.arange(len(px)), idx)  # positions the scan did not re-simulate
+    rel_p = lambda p, d: float(np.mean(np.abs(p[pred, d] - plain[pred, d]) / np.abs(plain[pred, d])))
     errs = {S.DET_NAMES[d]: {"surrogate_scan": round(rel(out, d), 5), "network_only": round(rel(net_only, d), 5),
-                             "fallback_without_calibration": round(rel(raw, d), 5)}
+                             "fallback_without_calibration": round(rel(raw, d), 5),
+                             "predicted_positions_only": round(rel_p(out, d), 5)}
             for d in range(3)}
     log(f"plain {t_plain:.2f} s, surrogate {t_surr:.2f} s ({tm}); re-simulated {len(idx)} of {len(px)}; "
         f"errors {errs}")
@@ -483,7 +506,9 @@ def main():
     ap.add_argument("--positions-per-step", type=int, default=512)
     ap.add_argument("--batch", type=int, default=2048, help="probe passes resident per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="jobs64: replay batches from HIP graphs")
+    ap.add_argument("--graph", action="store_true", help="jobs64 with --streams: replay batches from HIP graphs")
+    ap.add_argument("--streams", action="store_true",
+                    help="jobs64: one engine launch sequence per job on its own stream instead of grouped launches")
     ap.add_argument("--no-graph", action="store_true",
                     help="stem512: time eager launches instead of HIP graph replays")
     ap.add_argument("--workload", choices=["stem512", "thick2048", "jobs64", "surrogate"], default="stem512",
@@ -492,15 +517,15 @@ def main():
                          "train the detector-intensity surrogate and time surrogate+fallback vs plain")
     ap.add_argument("--lattices", type=int, default=500, help="surrogate: training lattices")
     ap.add_argument("--train-positions", type=int, default=400, help="surrogate: positions per lattice")
-    ap.add_argument("--epochs", type=int, default=30, help="surrogate: training epochs")
+    ap.add_argument("--epochs", type=int, default=40, help="surrogate: training epochs")
     ap.add_argument("--scan", type=int, default=256, help="surrogate: evaluation scan is scan x scan")
     ap.add_argument("--lr", type=float, default=1e-3, help="surrogate: initial learning rate")
-    ap.add_argument("--augment", action="store_true",
-                    help="surrogate: train on the 8 symmetry (4mm) images of every training position")
-    ap.add_argument("--sym", action="store_true",
-                    help="surrogate: average each prediction over the 8 symmetry images of the position")
+    ap.add_argument("--no-augment", dest="augment", action="store_false",
+                    help="surrogate: train on the positions as drawn (default: also on their 8 symmetry (4mm) images)")
+    ap.add_argument("--no-sym", dest="sym", action="store_false",
+                    help="surrogate: one prediction per position (default: averaged over its 8 symmetry images)")
     ap.add_argument("--fma", action="store_true", help="surrogate: FMA convolution kernels instead of MFMA")
-    ap.add_argument("--calib", type=int, default=0, metavar="K",
+    ap.add_argument("--calib", type=int, default=6, metavar="K",
                     help="surrogate: calibrate the predictions with the re-simulated positions (periodic 4mm "
                          "basis up to order K; 0 = off)")
     ap.add_argument("--window-step", type=float, default=0.1,

@@ -137,6 +137,29 @@ int jms_hip_sync(jms_hip_engine* e);
  * allocating library call. */
 int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* waves, const float* ppar,
                               const int* var, int npos, int npass, const float* weights, float* res);
+/* Uneven batch in grouped launches (no counterpart in the reference, which
+ * runs one probe at a time, JMultiSlice.cpp:4590): a group holds n engines
+ * with the same grid and device, each set up for its own specimen, probe
+ * and detectors and with its own staged scan.  jms_hip_group_run runs every
+ * member's staged scan in full, as jms_hip_run_staged(e, 0, npos, 0) on each
+ * would (the same results to float32 rounding: the grouped kernels are
+ * instantiations of their own), but the waves of all members advance
+ * together: one row-pass and one column-pass launch per slice step over the
+ * waves of the members that still have slices (plus one column launch for
+ * the members whose exit plane it is), driven by per-wave, per-member and
+ * per-plane descriptor tables.  Enqueued on the first member's stream (in
+ * the order given); every member's stream waits for the group, so
+ * jms_hip_sync on any member waits for it.  Members with more positions
+ * than one batch (jms_hip_set_batch) continue in further rounds.  Integrated
+ * detectors only; grids whose column pass is the wave-local kernel (edges up
+ * to 768).  create: 2 invalid list, 3 an engine twice, 4 grids or devices
+ * differ (or no grid), 5 grid not supported.  Destroy a group before its
+ * members. */
+typedef struct jms_hip_group jms_hip_group;
+int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, int n);
+int jms_hip_group_run(jms_hip_group* g);
+const char* jms_hip_group_last_error(jms_hip_group* g);
+void jms_hip_group_destroy(jms_hip_group* g);
 /* integrated results of the staged scan: npos*ndetslc*ndet floats,
  * [pos][idetslc][idet] */
 int jms_hip_get_scan_results(jms_hip_engine* e, float* out);

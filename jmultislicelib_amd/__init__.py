@@ -10,6 +10,6 @@ for gfx950; this package is its Python binding:
   * ``configs``      -- benchmark workloads (synthetic perovskite specimens)
 """
 from ._lib import EXPORTED, LIB_PATH, lib  # noqa: F401
-from .jms import Engine, JMSError, JMultiSlice, fft2d, grid_supported  # noqa: F401
+from .jms import Engine, Group, JMSError, JMultiSlice, fft2d, grid_supported  # noqa: F401
 
-__all__ = ["Engine", "JMultiSlice", "JMSError", "fft2d", "grid_supported", "lib", "LIB_PATH"]
+__all__ = ["Engine", "Group", "JMultiSlice", "JMSError", "fft2d", "grid_supported", "lib", "LIB_PATH"]

@@ -115,6 +115,10 @@ _SIGS = [
     ("jms_hip_sync", c_int, [c_void_p]),
     ("jms_hip_run_batch_noalloc", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                           c_void_p, c_void_p]),
+    ("jms_hip_group_create", c_int, [c_void_p, c_void_p, c_int]),
+    ("jms_hip_group_run", c_int, [c_void_p]),
+    ("jms_hip_group_last_error", c_char_p, [c_void_p]),
+    ("jms_hip_group_destroy", None, [c_void_p]),
     ("jms_hip_get_scan_results", c_int, [c_void_p, c_void_p]),
     ("jms_hip_set_batch", c_int, [c_void_p, c_int]),
     ("jms_hip_set_graph", c_int, [c_void_p, c_int]),

@@ -78,6 +78,15 @@ struct CfgP {
   static constexpr bool GTW_ROW = Tuned<N>::GTW;
   // fused tile I/O in the wave-local column kernel where the plan allows it
   static constexpr bool FIO = true;
+  // XCD-aware block order (block_xy) in the row pass: where a slice's
+  // grating (8 n bytes, 32 MiB at N = 2048) is far larger than one XCD's
+  // 4 MiB L2, the waves sharing a grating row then run on one XCD back to
+  // back (2048^2, 32 waves: 0.453 -> 0.427 ms per launch on MI355X).  Off in
+  // the long-column kernel: there the blocks in flight on an XCD would all
+  // read the same column tile of different waves, addresses a whole wave
+  // (32 MiB) apart, and the launch slowed from 0.755 to 0.819 ms.
+  static constexpr bool XCD_ROW = N >= 1024;
+  static constexpr bool XCD_COL = false;
 };
 
 // the configuration the library is built with

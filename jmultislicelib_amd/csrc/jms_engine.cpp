@@ -946,6 +946,285 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
                        BatchChans(), false);
 }
 
+// ---------------------------------------------------------------------------
+// Grouped launches over several engines with one grid (the uneven batch).
+// Every member's staged scan runs in full; the waves of all members advance
+// together, one row and one column launch per slice step over the waves of
+// the members that still have slices (members sorted by slice count, so
+// those are a prefix of the wave map), the members that reach their exit
+// plane at a step in a launch of their own.  Tables:
+//   plane table [member][maxS + 1] (GrpPlane): slice offset and propagator
+//     of each step, readout offset of each plane;
+//   per round: job table [member] (GrpJob) and wave map (GrpWave).
+// A round takes at most one batch of each member (auto_batch); members with
+// more positions than one batch continue in later rounds.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct jms_hip_group {
+  std::vector<jms_hip_engine*> eng;  // members, by slice count descending
+  int device = 0, nx = 0, ny = 0, smax = 0;
+  std::string err;
+  struct Round {
+    size_t job_off = 0, wav_off = 0;  // byte offsets in tab
+    int nwave = 0;
+    std::vector<int> active;          // active[p]: waves whose member has > p slices (p = 0..smax)
+    std::vector<int> member_w;        // waves of each member in this round
+    std::vector<int> member_p0;       // first position of each member in this round
+    std::vector<std::vector<char>> det;  // det[p][member]: member reads plane p out
+  };
+  std::vector<Round> rounds;
+  std::vector<char> host;             // tables as last uploaded
+  DevBuf tab;
+  hipEvent_t ev_done = nullptr;
+  std::vector<hipEvent_t> ev_in;
+  int fail(int code, const char* what, hipError_t e = hipSuccess) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s%s%s", what, e == hipSuccess ? "" : ": ", e == hipSuccess ? "" : hipGetErrorString(e));
+    err = buf;
+    fprintf(stderr, "jms_hip error: group: %s\n", buf);
+    return code;
+  }
+};
+
+namespace {
+
+// host tables of the group for the members' current state; fills g->rounds
+int group_build(jms_hip_group* g, std::vector<char>& host) {
+  const int m = (int)g->eng.size();
+  g->smax = 0;
+  for (auto* e : g->eng) g->smax = std::max(g->smax, e->nobjslc);
+  const int ld = g->smax + 1;
+  std::vector<int> bpos(m), nround(m);
+  int R = 0;
+  for (int k = 0; k < m; ++k) {
+    jms_hip_engine* e = g->eng[k];
+    const int total = e->s_npos * e->s_npass;
+    int bw = auto_batch(e, total);
+    bpos[k] = std::max(1, bw / e->s_npass);
+    int rc = ensure_batch(e, bpos[k] * e->s_npass);
+    if (rc) return g->fail(110, "allocate member batch");
+    nround[k] = (e->s_npos + bpos[k] - 1) / bpos[k];
+    R = std::max(R, nround[k]);
+  }
+  // plane table
+  std::vector<jms::GrpPlane> planes((size_t)m * ld);
+  for (int k = 0; k < m; ++k) {
+    jms_hip_engine* e = g->eng[k];
+    const int ntc = e->ntiles_col();
+    const int ndet = (e->detflags & JMS_HIP_DETECT_INTEGRATED) ? e->ndet : 0;
+    for (int p = 0; p <= e->nobjslc; ++p) {
+      jms::GrpPlane& q = planes[(size_t)k * ld + p];
+      if (p < e->nobjslc) {
+        const int js = e->objslc[p] % e->nscslc;
+        q.slc_off = e->slcoffset[js];
+        q.pro_t = e->pro_t.as<float2>() + (long long)e->proidx[js] * e->n;
+      }
+      const int id = e->det_objslc[p];
+      q.part_off = (id >= 0 && ndet > 0) ? (long long)id * ndet * ntc : -1;
+    }
+  }
+  size_t off = sizeof(jms::GrpPlane) * planes.size();
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  off = align(off);
+  g->rounds.assign(R, jms_hip_group::Round());
+  std::vector<std::vector<jms::GrpJob>> jobs(R);
+  std::vector<std::vector<jms::GrpWave>> wmap(R);
+  for (int r = 0; r < R; ++r) {
+    auto& rd = g->rounds[r];
+    rd.member_w.assign(m, 0);
+    rd.member_p0.assign(m, 0);
+    rd.active.assign(ld, 0);
+    rd.det.assign(ld, std::vector<char>(m, 0));
+    jobs[r].resize(m);
+    for (int k = 0; k < m; ++k) {
+      jms_hip_engine* e = g->eng[k];
+      const int p0 = r * bpos[k];
+      const int np = std::max(0, std::min(bpos[k], e->s_npos - p0));
+      const int nw = np * e->s_npass;
+      const int S = e->nobjslc;
+      jms::GrpJob& jb = jobs[r][k];
+      jb.wave = e->waves.p;
+      jb.src = e->wav0.p;
+      jb.ppar = e->s_ppar.as<float4>() + (long long)p0 * e->s_npass;
+      jb.var = e->s_var.as<int>() + (long long)p0 * e->s_npass * S;
+      jb.partial = e->partial.as<float>();
+      jb.pgr = e->pgr.p;
+      jb.detw_t = e->detw_t.as<float>();
+      jb.cppl_t = e->have_cppl ? e->cppl_t.as<float>() : nullptr;
+      jb.kscx = 1.f / e->a0x;
+      jb.kscy = 1.f / e->a0y;
+      jb.nslc = S;
+      jb.ndet = (e->detflags & JMS_HIP_DETECT_INTEGRATED) ? e->ndet : 0;
+      jb.part_wstride = (long long)e->ndetslc * e->ndet * e->ntiles_col();
+      rd.member_w[k] = nw;
+      rd.member_p0[k] = p0;
+      for (int w = 0; w < nw; ++w) wmap[r].push_back({k, w});
+      for (int p = 0; p < ld; ++p) {
+        if (S > p) rd.active[p] += nw;
+        if (nw > 0 && p <= S && planes[(size_t)k * ld + p].part_off >= 0) rd.det[p][k] = 1;
+      }
+    }
+    rd.nwave = (int)wmap[r].size();
+    rd.job_off = off;
+    off = align(off + sizeof(jms::GrpJob) * m);
+    rd.wav_off = off;
+    off = align(off + sizeof(jms::GrpWave) * std::max<size_t>(1, wmap[r].size()));
+  }
+  host.assign(off, 0);
+  memcpy(host.data(), planes.data(), sizeof(jms::GrpPlane) * planes.size());
+  for (int r = 0; r < R; ++r) {
+    memcpy(host.data() + g->rounds[r].job_off, jobs[r].data(), sizeof(jms::GrpJob) * m);
+    if (!wmap[r].empty())
+      memcpy(host.data() + g->rounds[r].wav_off, wmap[r].data(), sizeof(jms::GrpWave) * wmap[r].size());
+  }
+  return 0;
+}
+
+// any member among the waves [w0, w1) of round rd reads plane p out
+bool group_detects(const jms_hip_group* g, const jms_hip_group::Round& rd, int p, int w0, int w1) {
+  int base = 0;
+  for (size_t k = 0; k < g->eng.size(); ++k) {
+    const int nw = rd.member_w[k];
+    if (nw > 0 && base < w1 && base + nw > w0 && rd.det[p][k]) return true;
+    base += nw;
+  }
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, int n) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (!engines || n < 1) return 2;
+  for (int k = 0; k < n; ++k) {
+    if (!engines[k]) return 2;
+    for (int j = 0; j < k; ++j)
+      if (engines[j] == engines[k]) return 3;  // a member twice
+  }
+  jms_hip_engine* e0 = engines[0];
+  if (e0->n <= 0) return 4;
+  for (int k = 0; k < n; ++k) {
+    jms_hip_engine* e = engines[k];
+    if (e->nx != e0->nx || e->ny != e0->ny || e->device != e0->device) return 4;
+  }
+  if (!jms::grp_supported(e0->nx, e0->ny)) return 5;
+  jms_hip_group* g = new jms_hip_group();
+  g->eng.assign(engines, engines + n);
+  std::stable_sort(g->eng.begin(), g->eng.end(),
+                   [](const jms_hip_engine* a, const jms_hip_engine* b) { return a->nobjslc > b->nobjslc; });
+  g->device = e0->device;
+  g->nx = e0->nx;
+  g->ny = e0->ny;
+  (void)hipSetDevice(g->device);
+  if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess) { delete g; return 101; }
+  g->ev_in.resize(n, nullptr);
+  for (int k = 0; k < n; ++k)
+    if (hipEventCreateWithFlags(&g->ev_in[k], hipEventDisableTiming) != hipSuccess) {
+      for (auto ev : g->ev_in) if (ev) (void)hipEventDestroy(ev);
+      (void)hipEventDestroy(g->ev_done);
+      delete g;
+      return 101;
+    }
+  *out = g;
+  return 0;
+}
+
+void jms_hip_group_destroy(jms_hip_group* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (!g->eng.empty()) (void)hipStreamSynchronize(g->eng[0]->st);
+  g->tab.release();
+  for (auto ev : g->ev_in) if (ev) (void)hipEventDestroy(ev);
+  if (g->ev_done) (void)hipEventDestroy(g->ev_done);
+  delete g;
+}
+
+const char* jms_hip_group_last_error(jms_hip_group* g) { return g ? g->err.c_str() : "null group"; }
+
+int jms_hip_group_run(jms_hip_group* g) {
+  if (!g) return 1;
+  for (auto* e : g->eng) {
+    int rc = check_ready(e);
+    if (rc) return g->fail(10 + rc, "member not ready");
+    if (!e->wav0.p) return g->fail(16, "member probe not set");
+    if (e->s_npos < 1) return g->fail(17, "member has no staged scan");
+  }
+  (void)hipSetDevice(g->device);
+  jms_hip_engine* lead = g->eng[0];
+  hipStream_t st = lead->st;
+  // the group's launches follow every member's earlier work
+  for (size_t k = 1; k < g->eng.size(); ++k) {
+    hipError_t err = hipEventRecord(g->ev_in[k], g->eng[k]->st);
+    if (err == hipSuccess) err = hipStreamWaitEvent(st, g->ev_in[k], 0);
+    if (err != hipSuccess) return g->fail(102, "order member streams", err);
+  }
+  std::vector<char> host;
+  int rc = group_build(g, host);
+  if (rc) return rc;
+  if (host != g->host) {  // tables changed: upload once no launch reads the old ones
+    hipError_t err = hipStreamSynchronize(st);
+    if (err == hipSuccess) err = g->tab.ensure(host.size());
+    if (err == hipSuccess) err = hipMemcpy(g->tab.p, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) return g->fail(103, "upload group tables", err);
+    g->host.swap(host);
+  }
+  const char* base = (const char*)g->tab.p;
+  const long long n = (long long)g->nx * g->ny;
+  const int ld = g->smax + 1;
+  const int ntc = lead->ntiles_col();
+  hipError_t err = hipSuccess;
+  for (const auto& rd : g->rounds) {
+    if (rd.nwave == 0) continue;
+    jms::GrpArgs ga;
+    ga.wav = (const jms::GrpWave*)(base + rd.wav_off);
+    ga.job = (const jms::GrpJob*)(base + rd.job_off);
+    ga.pl = (const jms::GrpPlane*)base;
+    ga.pl_ld = ld;
+    // INIT column pass: plane 0 for every wave; members without slices exit
+    auto col = [&](int plane, bool init, int w0, int w1, bool exit_) -> hipError_t {
+      if (w1 <= w0) return hipSuccess;
+      ga.w0 = w0;
+      ga.step = plane;
+      TimedLaunch tl(lead, 1, lead->timing);
+      return jms::launch_col_pass_grp(g->nx, g->ny, w1 - w0, init, exit_, group_detects(g, rd, plane, w0, w1), ntc,
+                                      lead->twy.as<float2>(), ga, st);
+    };
+    err = col(0, true, 0, rd.active[0], false);
+    if (err == hipSuccess) err = col(0, true, rd.active[0], rd.nwave, true);
+    for (int s = 0; err == hipSuccess && s < g->smax && rd.active[s] > 0; ++s) {
+      ga.w0 = 0;
+      ga.step = s;
+      {
+        TimedLaunch tl(lead, 0, lead->timing);
+        err = jms::launch_row_pass_grp(g->nx, g->ny, rd.active[s], n, n, lead->twx.as<float2>(), ga, st);
+      }
+      if (err == hipSuccess) err = col(s + 1, false, 0, rd.active[s + 1], false);
+      if (err == hipSuccess) err = col(s + 1, false, rd.active[s + 1], rd.active[s], true);
+    }
+    if (err != hipSuccess) return g->fail(131, "grouped launch", err);
+    for (size_t k = 0; k < g->eng.size(); ++k) {
+      jms_hip_engine* e = g->eng[k];
+      const int nw = rd.member_w[k];
+      if (nw == 0 || !(e->detflags & JMS_HIP_DETECT_INTEGRATED) || e->ndet < 1) continue;
+      const int per = e->ndetslc * e->ndet;
+      TimedLaunch tl(lead, 2, lead->timing);
+      err = jms::launch_combine(e->partial.as<float>(), ntc, e->ndet, e->ndetslc, nw / e->s_npass, e->s_npass,
+                                e->s_weights.as<float>(), e->s_res.as<float>() + (long long)rd.member_p0[k] * per, 0,
+                                st);
+      if (err != hipSuccess) return g->fail(133, "combine", err);
+    }
+  }
+  // members' later work (and jms_hip_sync on any member) follows the group
+  err = hipEventRecord(g->ev_done, st);
+  for (size_t k = 1; err == hipSuccess && k < g->eng.size(); ++k) err = hipStreamWaitEvent(g->eng[k]->st, g->ev_done, 0);
+  if (err != hipSuccess) return g->fail(104, "order member streams", err);
+  return 0;
+}
+
 int jms_hip_sync(jms_hip_engine* e) {
   if (!e) return 1;
   (void)hipSetDevice(e->device);

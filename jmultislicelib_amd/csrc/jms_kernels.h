@@ -79,6 +79,56 @@ struct FFAtom {
   int type;
 };
 
+// Grouped launches (the uneven batch): the waves of several jobs with one
+// grid advance together, one launch per pass and step over all of them.  A
+// launch's block row y takes wave w0 + y of the wave map, which names its job
+// and its index in that job's batch; the job's constant buffers and the
+// per-plane slice data come from the job and plane tables.
+struct GrpJob {
+  void* wave = nullptr;           // the job's batch of waves (float2, nx*ny apart)
+  const void* src = nullptr;      // INIT source: the job's probe (float2)
+  const void* ppar = nullptr;     // float4 (dx, dy, fz, 0) per wave
+  const int* var = nullptr;       // variant indices [wave][nslc]
+  float* partial = nullptr;       // readout partial sums [wave][plane][detector][tile]
+  const void* pgr = nullptr;      // phase gratings (float2)
+  const float* detw_t = nullptr;  // [ndet][nx][ny] detector weights
+  const float* cppl_t = nullptr;  // constant phase plate [x][y] or nullptr
+  float kscx = 0.f, kscy = 0.f;   // 1/a0x, 1/a0y [1/nm]
+  int nslc = 0;                   // slices of the job (row length of var)
+  int ndet = 0;
+  long long part_wstride = 0;     // per wave in partial
+};
+// plane p of a job: slice data of step p (p < nslc) and the readout of
+// plane p (made by the column pass of step p - 1, or the INIT pass for p = 0)
+struct GrpPlane {
+  long long slc_off = 0;          // grating offset of the slice of step p (variant 0)
+  const void* pro_t = nullptr;    // propagator of step p, column-major (float2)
+  long long part_off = -1;        // offset of plane p in partial; -1: no readout
+};
+struct GrpWave {
+  int job = 0;                    // index in the job table
+  int w = 0;                      // wave index in that job's batch
+};
+struct GrpArgs {
+  const GrpWave* wav = nullptr;   // wave map
+  const GrpJob* job = nullptr;    // job table
+  const GrpPlane* pl = nullptr;   // plane table [job][pl_ld]
+  int pl_ld = 0;
+  int w0 = 0;                     // first wave of the launch in the wave map
+  int step = 0;                   // row pass: slice step; column pass: plane read out
+};
+// grouped launches exist for grids whose column pass is the wave-local kernel
+bool grp_supported(int nx, int ny);
+// row pass of step ga.step over nwave waves of the map (from ga.w0)
+hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride, long long npg,
+                               const float2* twx, const GrpArgs& ga, hipStream_t st);
+// column pass producing plane ga.step (propagator of step ga.step - 1 unless
+// init) over nwave waves of the map; detect: some job of the launch reads
+// plane ga.step out (each wave checks its own job's plane table)
+hipError_t launch_col_pass_grp(int nx, int ny, int nwave, bool init, bool exit_, bool detect,
+                               long long part_dstride, const float2* twy, const GrpArgs& ga,
+                               hipStream_t st);
+
 constexpr int kMaxDetPerSweep = 8;
 
 bool size_supported(int n);

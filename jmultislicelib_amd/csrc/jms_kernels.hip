@@ -26,6 +26,12 @@ hipError_t launch_row_fft_n(int ny, int nbatch, float2* data, long long stride, 
 template <int N>
 hipError_t launch_col_fft_n(int nx, int nbatch, float2* data, long long stride, int dir,
                             const float2* twy, hipStream_t st);
+template <int N>
+hipError_t launch_row_pass_grp_n(int ny, int nwave, long long wave_stride, long long npg, const float2* twx,
+                                 const GrpArgs& ga, hipStream_t st);
+template <int N>
+hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool detect, long long part_dstride,
+                                 const float2* twy, const GrpArgs& ga, hipStream_t st);
 
 // frequency index of position i on an n-point grid in FFT order
 // (JMultiSlice.cpp:1699-1712)
@@ -370,6 +376,38 @@ hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColAr
                            hipStream_t st) {
   switch (ny) {
 #define JMS_CASE(N) case N: return launch_col_pass_n<N>(nwave, init, exit_, a, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool grp_supported(int nx, int ny) {
+  if (!size_supported(nx)) return false;
+  switch (ny) {
+#define JMS_CASE(N) case N: return Cfg<N>::WL && !Cfg<N>::XL;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return false;
+  }
+}
+
+hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride, long long npg,
+                               const float2* twx, const GrpArgs& ga, hipStream_t st) {
+  switch (nx) {
+#define JMS_CASE(N) case N: return launch_row_pass_grp_n<N>(ny, nwave, wave_stride, npg, twx, ga, st);
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_col_pass_grp(int nx, int ny, int nwave, bool init, bool exit_, bool detect,
+                               long long part_dstride, const float2* twy, const GrpArgs& ga,
+                               hipStream_t st) {
+  switch (ny) {
+#define JMS_CASE(N) \
+  case N: return launch_col_pass_grp_n<N>(nx, nwave, init, exit_, detect, part_dstride, twy, ga, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;

@@ -66,6 +66,29 @@ __device__ __forceinline__ int fidx(int i, int n) {
   return ((i + h) % n) - h;
 }
 
+// Block coordinates (bx, by) of a launch.  The hardware hands block
+// b = blockIdx.x + gridDim.x * blockIdx.y to XCD b % 8, and each of the 8
+// XCDs has its own L2.  With XCD, the blocks one XCD receives take
+// consecutive logical ids and the logical id runs over by fastest, so an XCD
+// works through all block rows by (waves) of a block column bx back to back:
+// the grating row or propagator tile of bx enters that XCD's L2 once and
+// serves every wave.  A bijection of the grid for any size; identity
+// without XCD.
+template <bool XCD>
+__device__ __forceinline__ void block_xy(int& bx, int& by) {
+  if constexpr (XCD) {
+    const unsigned nbx = gridDim.x, nby = gridDim.y, nb = nbx * nby;
+    const unsigned b = blockIdx.x + nbx * blockIdx.y;
+    const unsigned x8 = b & 7u, q = nb >> 3, r = nb & 7u;
+    const unsigned id = x8 * q + (x8 < r ? x8 : r) + (b >> 3);
+    bx = (int)(id / nby);
+    by = (int)(id % nby);
+  } else {
+    bx = (int)blockIdx.x;
+    by = (int)blockIdx.y;
+  }
+}
+
 // real-space readout channels of one detection plane (per wave: weight
 // cwgt[w], channel stride chan_wstride items, plane offset chan_off items)
 struct RowChan {
@@ -87,13 +110,17 @@ struct RowChan {
 //           Fourier-space wave (the column pass did IFFT_y);
 //   MODE 2: IFFT_x and the readout only (exit plane after a column IFFT);
 //           the wave is not written back.
+//   GRP:    grouped launch (MODE 0): block row y is wave ga.w0 + y of the
+//           wave map; wave buffer, variant table, gratings and the slice of
+//           step jslc come from its job's tables.
 // grid: (ceil(ny/LPB), nwave); block: T*LPB
 // ---------------------------------------------------------------------------
-template <int N, int MODE = 0, class C_ = Cfg<N>>
+template <int N, int MODE = 0, class C_ = Cfg<N>, bool GRP = false>
 __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu(C_::WPE_ROW))) void k_row_pass(
     float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
     long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
-    const float2* __restrict__ twg, RowChan rc) {
+    const float2* __restrict__ twg, RowChan rc, GrpArgs ga) {
+  static_assert(!GRP || MODE == 0, "grouped row passes carry no real-space readout");
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH, C_::SW>;
@@ -107,8 +134,20 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   if constexpr (!C_::GTW_ROW)
     for (int e = tid; e < N; e += T * LPB) s_tw[e] = twg[e];
   const float2* tw = C_::GTW_ROW ? twg : s_tw;
-  const long long w = blockIdx.y;
-  const int row = blockIdx.x * LPB + line;
+  int bx, by;
+  block_xy<C_::XCD_ROW>(bx, by);
+  long long w = by;
+  if constexpr (GRP) {  // this wave's job: its buffers and the slice of step jslc
+    const GrpWave gw = ga.wav[ga.w0 + by];
+    const GrpJob& jb = ga.job[gw.job];
+    w = gw.w;
+    wave = (float2*)jb.wave;
+    var_tab = jb.var;
+    var_ld = jb.nslc;
+    pgr = (const float2*)jb.pgr;
+    slc_off = ga.pl[(long long)gw.job * ga.pl_ld + jslc].slc_off;
+  }
+  const int row = bx * LPB + line;
   const bool valid = row < ny;
   float2* wrow = wave + w * wave_stride + (long long)row * N;
   const int var = MODE == 2 ? 0 : var_tab[w * var_ld + jslc];
@@ -251,6 +290,7 @@ struct ColArgs {
   const float2* pro_t;      // propagator of this slice
   const float* detw_t;      // [ndet][nx][ny] detector weights
   const float* cppl_t;      // constant phase plate
+  GrpArgs grp;              // grouped launch: wave map and job / plane tables
 };
 
 template <int N, bool INIT, bool EXIT, class C_ = Cfg<N>>
@@ -555,7 +595,9 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
       }
     });
   }
-  if constexpr (DET) {
+  // a.detect is uniform over the block (one wave per block row): in a grouped
+  // launch, jobs that do not read this plane out skip the readout
+  if (DET && a.detect) {
     float acc[kMaxDet];
 #pragma unroll
     for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
@@ -663,8 +705,9 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
 // cppl_t) so that the lanes of a column access them contiguously.
 // grid: (ceil(nx/CW), nwave); block: TW*CW
 // ---------------------------------------------------------------------------
-template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
-__global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(ColArgs a) {
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool GRP = false>
+__global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(
+    ColArgs a0) {
   constexpr int T = C_::TW, C = C_::CW, BS = T * C;
   static_assert(64 % T == 0, "a column must lie inside one wavefront");
   static_assert(BS % 64 == 0, "block must be whole wavefronts");
@@ -677,8 +720,33 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   const int lane = tid & 63, wv = tid >> 6;
   const int c = wv * (64 / T) + lane / T;  // column within the tile
   const int t = lane % T;
-  for (int e = tid; e < N; e += BS) s_tw[e] = a.twg[e];
-  const long long w = blockIdx.y;
+  for (int e = tid; e < N; e += BS) s_tw[e] = a0.twg[e];
+  // GRP: block row y is wave grp.w0 + y of the wave map; its job supplies the
+  // buffers, and the plane table the propagator and the readout of plane
+  // grp.step (each job reads out on its own planes)
+  ColArgs a = a0;
+  long long w = blockIdx.y;
+  if constexpr (GRP) {
+    const GrpWave gw = a0.grp.wav[a0.grp.w0 + blockIdx.y];
+    const GrpJob& jb = a0.grp.job[gw.job];
+    const GrpPlane* pl = a0.grp.pl + (long long)gw.job * a0.grp.pl_ld;
+    w = gw.w;
+    a.wave = (float2*)jb.wave;
+    a.src = (const float2*)jb.src;
+    a.src_stride = 0;
+    a.ppar = (const float4*)jb.ppar;
+    a.kscx = jb.kscx;
+    a.kscy = jb.kscy;
+    a.cppl_t = jb.cppl_t;
+    a.ndet = jb.ndet;
+    a.detw_t = jb.detw_t;
+    a.partial = jb.partial;
+    a.part_wstride = jb.part_wstride;
+    const long long po = pl[a0.grp.step].part_off;
+    a.detect = po >= 0 ? 1 : 0;
+    a.part_off = po >= 0 ? po : 0;
+    if constexpr (!INIT) a.pro_t = (const float2*)pl[a0.grp.step - 1].pro_t;
+  }
   const int nx = a.nx;
   const int x0 = blockIdx.x * C;
   const int ncol = (nx - x0 < C) ? (nx - x0) : C;
@@ -793,9 +861,11 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   __shared__ float s_red[16 * kMaxDet];
   const int tid = threadIdx.x;
   for (int e = tid; e < N; e += 1024) s_tw[e] = a.twg[e];
-  const long long w = blockIdx.y;
+  int bx, by;
+  block_xy<C_::XCD_COL>(bx, by);
+  const long long w = by;
   const int nx = a.nx;
-  const int x0 = blockIdx.x * CT;
+  const int x0 = bx * CT;
   const int ncol = (nx - x0 < CT) ? (nx - x0) : CT;
   float2* wbase = a.wave + w * a.wave_stride + x0;
   // A layout (columns past the grid edge work on column 0 of the tile and
@@ -938,8 +1008,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
     }
   });
   if constexpr (DET)
-    block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + blockIdx.x,
-                     a.part_dstride);
+    block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + bx, a.part_dstride);
   if constexpr (!EXIT) {
     static_for<0, R>([&](auto RR) {
       constexpr int r = decltype(RR)::value;
@@ -1087,13 +1156,23 @@ hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_str
   rc.cwgt = h.cwgt; rc.chan_wstride = h.chan_wstride; rc.chan_off = h.chan_off;
   if (mode == 1)
     hipLaunchKernelGGL((k_row_pass<N, 1>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
-                       var_tab, var_ld, jslc, twx, rc);
+                       var_tab, var_ld, jslc, twx, rc, GrpArgs{});
   else if (mode == 2)
     hipLaunchKernelGGL((k_row_pass<N, 2>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
-                       var_tab, var_ld, jslc, twx, rc);
+                       var_tab, var_ld, jslc, twx, rc, GrpArgs{});
   else
     hipLaunchKernelGGL((k_row_pass<N, 0>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
-                       var_tab, var_ld, jslc, twx, rc);
+                       var_tab, var_ld, jslc, twx, rc, GrpArgs{});
+  return hipGetLastError();
+}
+
+// grouped row pass (waves of several jobs, one slice step)
+template <int N>
+hipError_t launch_row_pass_grp_n(int ny, int nwave, long long wave_stride, long long npg, const float2* twx,
+                                 const GrpArgs& ga, hipStream_t st) {
+  dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nwave), b(Cfg<N>::T * Cfg<N>::LPB);
+  hipLaunchKernelGGL((k_row_pass<N, 0, Cfg<N>, true>), g, b, 0, st, nullptr, ny, wave_stride, nullptr, 0LL, npg,
+                     nullptr, 0, ga.step, twx, RowChan{}, ga);
   return hipGetLastError();
 }
 
@@ -1151,6 +1230,36 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
     else hipLaunchKernelGGL((k_col_pass<N, false, false>), g, b, 0, st, a);
   }
   return hipGetLastError();
+}
+
+// grouped column pass (waves of several jobs; the wave-local kernel only)
+template <int N>
+hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool detect, long long part_dstride,
+                                 const float2* twy, const GrpArgs& ga, hipStream_t st) {
+  using CF = Cfg<N>;
+  if constexpr (CF::WL && !CF::XL) {
+    ColArgs a{};
+    a.nx = nx;
+    a.wave_stride = (long long)N * nx;
+    a.part_dstride = part_dstride;
+    a.twg = twy;
+    a.grp = ga;
+    dim3 g((nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
+    if (detect) {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, true, CF, true>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, true, CF, true>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, true, CF, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, true, CF, true>), g, b, 0, st, a);
+    } else {
+      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, false, CF, true>), g, b, 0, st, a);
+      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, false, CF, true>), g, b, 0, st, a);
+      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, false, CF, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, true>), g, b, 0, st, a);
+    }
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
 }
 
 template <int N>

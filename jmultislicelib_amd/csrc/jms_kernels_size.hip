@@ -11,6 +11,10 @@ template hipError_t launch_row_pass_n<JMS_N>(int, int, float2*, long long, const
                                              long long, const int*, int, int, const float2*,
                                              const RowChanHost&, int, hipStream_t);
 template hipError_t launch_col_pass_n<JMS_N>(int, bool, bool, const ColArgsHost&, hipStream_t);
+template hipError_t launch_row_pass_grp_n<JMS_N>(int, int, long long, long long, const float2*, const GrpArgs&,
+                                                 hipStream_t);
+template hipError_t launch_col_pass_grp_n<JMS_N>(int, int, bool, bool, bool, long long, const float2*,
+                                                 const GrpArgs&, hipStream_t);
 template hipError_t launch_row_fft_n<JMS_N>(int, int, float2*, long long, int, const float2*,
                                             hipStream_t);
 template hipError_t launch_col_fft_n<JMS_N>(int, int, float2*, long long, int, const float2*,

@@ -403,9 +403,9 @@ void release(jms_nn* m) {
 }
 
 // matrix-core convolution of layer l (one block per sample)
-void conv_fwd_mfma(jms_nn* m, int l, int n) {
+void conv_fwd_mfma(jms_nn* m, int l, int n, const uint16_t* in0) {
   const dim3 g(n), b(256);
-  const uint16_t* x = m->act[l];
+  const uint16_t* x = (l == 0 && in0) ? in0 : m->act[l];
   const uint16_t* w = m->pbf + off_cw(l);
   const float* bias = m->p + off_cb(l);
   uint16_t* y = m->act[l + 1];
@@ -446,18 +446,21 @@ void conv_bwd_mfma(jms_nn* m, int l, int n) {
 }
 
 // forward over act[0] (n samples) -> out
-hipError_t forward(jms_nn* m, int n) {
+// forward pass of n samples: input act[0] (or in0: n windows in place),
+// output [n][3] into out (default m->out)
+hipError_t forward(jms_nn* m, int n, const uint16_t* in0 = nullptr, float* out = nullptr) {
   for (int l = 0; l < NCONV; ++l) {
     if (m->mfma) {
-      conv_fwd_mfma(m, l, n);
+      conv_fwd_mfma(m, l, n, in0);
       continue;
     }
     const long long px = (long long)n * HOUT[l] * HOUT[l];
     const int gsz = group_size(px, CH);
     const dim3 g(nblk(px, 256), CH / gsz);
+    const uint16_t* x = (l == 0 && in0) ? in0 : m->act[l];
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, g, dim3(256), 0, m->st, m->act[l], m->pbf + off_cw(l), m->p + off_cb(l),
-                         m->act[l + 1], n, CIN[l], HIN[l], HOUT[l]);
+      hipLaunchKernelGGL(kern, g, dim3(256), 0, m->st, x, m->pbf + off_cw(l), m->p + off_cb(l), m->act[l + 1], n,
+                         CIN[l], HIN[l], HOUT[l]);
     };
     if (STRIDE[l] == 1) {
       if (gsz == 8) go(k_conv_fwd<1, 8>); else if (gsz == 4) go(k_conv_fwd<1, 4>);
@@ -470,7 +473,7 @@ hipError_t forward(jms_nn* m, int n) {
   hipLaunchKernelGGL(k_dense_fwd, dim3(nblk((long long)n * HID, 256)), dim3(256), 0, m->st, m->act[NCONV],
                      m->pbf + OFF_D1W, m->p + OFF_D1B, n, FLAT, HID, m->act[NCONV + 1], (float*)nullptr);
   hipLaunchKernelGGL(k_dense_fwd, dim3(nblk((long long)n * NOUT, 256)), dim3(256), 0, m->st, m->act[NCONV + 1],
-                     m->pbf + OFF_D2W, m->p + OFF_D2B, n, HID, NOUT, (uint16_t*)nullptr, m->out);
+                     m->pbf + OFF_D2W, m->p + OFF_D2B, n, HID, NOUT, (uint16_t*)nullptr, out ? out : m->out);
   return hipGetLastError();
 }
 
@@ -774,19 +777,20 @@ int jms_nn_train_epoch(jms_nn* m, jms_nn_data* d, int batch, float lr, const flo
 
 int jms_nn_predict(jms_nn* m, jms_nn_data* d, float* out) {
   if (!m || !d || !out) return 1;
-  std::vector<int> idx(m->maxb);
-  hipError_t e = hipSuccess;
+  if (d->n < 1) return 0;
+  // batches of up to max_batch windows read in place; every output goes to
+  // one device array, copied back once
+  float* dout = nullptr;
+  hipError_t e = dalloc(&dout, (size_t)d->n * NOUT);
   for (int b0 = 0; b0 < d->n && e == hipSuccess; b0 += m->maxb) {
     const int nb = std::min(m->maxb, d->n - b0);
-    for (int j = 0; j < nb; ++j) idx[j] = b0 + j;
-    e = hipMemcpyAsync(m->idx, idx.data(), nb * sizeof(int), hipMemcpyHostToDevice, m->st);
-    if (e != hipSuccess) break;
-    hipLaunchKernelGGL(k_gather, dim3(nblk((long long)nb * IN_SIZE, 256)), dim3(256), 0, m->st, d->x, d->t, m->idx,
-                       nb, m->act[0], m->tb);
-    e = forward(m, nb);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(out + (size_t)b0 * NOUT, m->out, (size_t)nb * NOUT * 4, hipMemcpyDeviceToHost, m->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(m->st);  // idx is reused by the next batch
+    e = forward(m, nb, d->x + (size_t)b0 * IN_SIZE, dout + (size_t)b0 * NOUT);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)d->n * NOUT * 4, hipMemcpyDeviceToHost, m->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+  if (dout) {
+    (void)hipStreamSynchronize(m->st);
+    (void)hipFree(dout);
   }
   return e == hipSuccess ? 0 : 2;
 }

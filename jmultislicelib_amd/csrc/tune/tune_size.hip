@@ -54,7 +54,7 @@ float time_row(const Bufs& b, int reps) {
   (void)hipEventRecord(e0);
   for (int i = 0; i < reps; ++i)
     hipLaunchKernelGGL((k_row_pass<N, 0, CF>), g, bl, 0, 0, b.w, N, (long long)N * N, b.pg, 0LL, (long long)N * N,
-                       b.var, 1, 0, b.tw, rc);
+                       b.var, 1, 0, b.tw, rc, GrpArgs{});
   (void)hipEventRecord(e1);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;

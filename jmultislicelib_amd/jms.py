@@ -458,6 +458,48 @@ class Engine:
         return out, w.value
 
 
+class Group:
+    """Grouped launches over engines with one grid (jms_hip_group): run()
+    runs every member's staged scan in full, the waves of all members
+    advancing through one row and one column launch per slice step.  The
+    results are read from each member (Engine.scan_results) as after
+    Engine.run_staged; Engine.sync on any member waits for the group."""
+
+    _CREATE_ERRORS = {2: "invalid engine list", 3: "an engine given twice", 4: "grids or devices differ",
+                      5: "grid not supported by the grouped kernels (column pass not wave-local)"}
+
+    def __init__(self, engines):
+        self.L = lib()
+        self.members = list(engines)
+        arr = (C.c_void_p * len(self.members))(*[e.h.value for e in self.members])
+        h = C.c_void_p()
+        rc = self.L.jms_hip_group_create(C.byref(h), arr, len(self.members))
+        if rc != 0:
+            raise JMSError(f"jms_hip_group_create ({self._CREATE_ERRORS.get(rc, '')})", rc)
+        self.h = h
+
+    def run(self):
+        rc = self.L.jms_hip_group_run(self.h)
+        if rc != 0:
+            msg = self.L.jms_hip_group_last_error(self.h)
+            raise JMSError(f"jms_hip_group_run ({msg.decode() if msg else ''})", rc)
+
+    def sync(self):
+        for e in self.members:
+            e.sync()
+
+    def close(self):
+        if self.h:
+            self.L.jms_hip_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fft2d(a: np.ndarray, direction: int) -> np.ndarray:
     """2-D FFT on the GPU of (..., ny, nx) complex64 data; direction < 0
     forward exp(-i), > 0 backward exp(+i), unscaled."""
