@@ -11,34 +11,41 @@
 
 namespace jms {
 
-// launch configuration of length N with FFT plan PL (the radix order; the
-// tuned plan of jms_tuned.h by default)
-template <int N, class PL = typename TunedPlan<N>::type>
+// launch configuration of length N with FFT plans PL for the row transforms
+// and PLC for the column transforms (radix orders; the tuned plans of
+// jms_tuned.h by default: the two passes are independent 1-D transforms, so
+// each may take the plan that suits its kernel)
+template <int N, class PL = typename TunedPlan<N>::type, class PLC = typename TunedColPlan<N>::type>
 struct CfgP {
   using P = PL;
-  static_assert(plan_product<P>() == N, "plan does not multiply to N");
-  static constexpr int max_bfly() {
+  using PC = PLC;
+  static_assert(plan_product<P>() == N, "row plan does not multiply to N");
+  static_assert(plan_product<PC>() == N, "column plan does not multiply to N");
+  template <class Q>
+  static constexpr int max_bfly_of() {
     int m = 0;
-    for (int s = 0; s < P::S; ++s) m = (N / P::R[s] > m) ? N / P::R[s] : m;
+    for (int s = 0; s < Q::S; ++s) m = (N / Q::R[s] > m) ? N / Q::R[s] : m;
     return m;
   }
-  static constexpr int T = max_bfly();                   // row pass: threads per line
+  static constexpr int T = max_bfly_of<P>();             // row pass: threads per line
+  static constexpr int TCOL = max_bfly_of<PC>();         // butterflies per column stage (at most)
   // rows per block: 256 threads' worth, or the tuned value (jms_tuned.h)
   // when it fits a block with this plan's threads per row
   static constexpr int LPB = (Tuned<N>::LPB > 0 && T * Tuned<N>::LPB <= 1024) ? Tuned<N>::LPB
                                                                                : ((256 / T) > 0 ? (256 / T) : 1);
   static constexpr int C = (N <= 640) ? 16 : ((N <= 1024) ? 8 : 4);  // columns per block
   static constexpr int TC0 = 1024 / C;
-  static constexpr int TC1 = (T < TC0) ? T : TC0;
+  static constexpr int TC1 = (TCOL < TC0) ? TCOL : TC0;
   // column pass: threads per line, rounded down so that a block is a whole
   // number of 64-lane wavefronts (the readout reduction shuffles over 64 lanes)
   static constexpr int TC = (C >= 64) ? TC1 : (TC1 / (64 / C)) * (64 / C);
   static_assert(TC >= 1 && (TC * C) % 64 == 0, "column block must be whole wavefronts");
   // wave-local column kernel: TW threads per column (largest power of two
-  // <= min(T, 64), so a column stays inside one wavefront), CW columns per
+  // <= min(T, 64), so a column stays inside one wavefront, unless the tuner
+  // chose fewer), CW columns per
   // block; used up to N = 768 (the staged tile must fit LDS)
   static constexpr int pow2_floor(int v) { int p = 1; while (2 * p <= v) p *= 2; return p; }
-  static constexpr int TW = pow2_floor(T < 64 ? T : 64);
+  static constexpr int TW = Tuned<N>::TW > 0 ? Tuned<N>::TW : pow2_floor(TCOL < 64 ? TCOL : 64);
   // 16 columns (128-byte segments) unless the tuner found 8 faster (where a
   // 16-column block cannot share its CU)
   static constexpr int CW = Tuned<N>::CW;

@@ -295,7 +295,7 @@ struct ColArgs {
 
 template <int N, bool INIT, bool EXIT, class C_ = Cfg<N>>
 __global__ __launch_bounds__(C_::TC * C_::C) void k_col_pass(ColArgs a) {
-  using P = typename C_::P;
+  using P = typename C_::PC;
   constexpr int T = C_::TC, C = C_::C;
   using L = TileLayout<N, C, C_::PADSH>;
   constexpr int NREG = regs_needed<N, T, P>();
@@ -525,12 +525,15 @@ struct ColTile {
 template <int N, bool INIT, bool EXIT, bool DET, class C_, bool FIN = false, bool FOUT = false>
 __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* s_buf, const float2* s_tw,
                                          float* s_red, int c, int t, int x, bool valid) {
-  using P = typename C_::P;
+  using P = typename C_::PC;
   constexpr int T = C_::TW;
   using L = ColLayout<N, C_::PADSH, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   using M0 = IOMap<N, T, P, false>;
   using M1 = IOMap<N, T, P, true>;
+  // a column inside one wavefront needs wave barriers only; a column over
+  // several wavefronts (T = 128) synchronises the block
+  using CS = std::conditional_t<(T <= 64), SyncWave, SyncBlock>;
   const int nx = a.nx;
   const long long n = (long long)N * nx;
   const long long colb = (long long)(valid ? x : x - c) * N;  // column-major offset
@@ -567,8 +570,8 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
   } else {
     if constexpr (FIN) {
       LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, -1, 1>::read_input(v, s_buf, c, t);
-      SyncWave{}();
-      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, -1, 1>::run(v, s_buf, c, s_tw, t, SyncWave{});
+      CS{}();
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, -1, 1>::run(v, s_buf, c, s_tw, t, CS{});
     } else {
       static_for<0, M0::NB>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -580,8 +583,8 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
           });
         }
       });
-      SyncWave{}();
-      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
+      CS{}();
+      LineFFT<N, T, -1, P, false, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, CS{});
     }
     const float2* pcol = a.pro_t + colb;
     static_for<0, M1::NB>([&](auto I) {
@@ -665,7 +668,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
     }
   }
   if constexpr (EXIT) {
-    SyncWave{}();
+    CS{}();
     static_for<0, M1::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
       const int b = t + i * T;
@@ -677,12 +680,12 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
       }
     });
   } else if constexpr (FOUT) {
-    SyncWave{}();
-    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR, P::S - 2>::run(v, s_buf, c, s_tw, t, SyncWave{});
+    CS{}();
+    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR, P::S - 2>::run(v, s_buf, c, s_tw, t, CS{});
   } else {
-    SyncWave{}();
-    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, SyncWave{});
-    SyncWave{}();
+    CS{}();
+    LineFFT<N, T, +1, P, true, L, NREG, C_::TWR>::run(v, s_buf, c, s_tw, t, CS{});
+    CS{}();
     static_for<0, M0::NB>([&](auto I) {
       constexpr int i = decltype(I)::value;
       const int b = t + i * T;
@@ -709,7 +712,7 @@ template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool GRP = f
 __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(
     ColArgs a0) {
   constexpr int T = C_::TW, C = C_::CW, BS = T * C;
-  static_assert(64 % T == 0, "a column must lie inside one wavefront");
+  static_assert(64 % T == 0 || T % 64 == 0, "a column is part of one wavefront or whole wavefronts");
   static_assert(BS % 64 == 0, "block must be whole wavefronts");
   using L = ColLayout<N, C_::PADSH, C_::SW>;
   using TL = ColTile<N, C_>;
@@ -717,9 +720,8 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   __shared__ float2 s_buf[L::words(C)];
   __shared__ float s_red[(BS / 64) * kMaxDet];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  const int c = wv * (64 / T) + lane / T;  // column within the tile
-  const int t = lane % T;
+  const int c = tid / T;  // column within the tile
+  const int t = tid % T;
   for (int e = tid; e < N; e += BS) s_tw[e] = a0.twg[e];
   // GRP: block row y is wave grp.w0 + y of the wave map; its job supplies the
   // buffers, and the plane table the propagator and the readout of plane
@@ -758,7 +760,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   // the exchange layout, and the last inverse stage (the same radix, reversed
   // plan) produces the rows that the store sweep writes.  This removes the
   // LDS round trips of staging in and out.
-  using P = typename C_::P;
+  using P = typename C_::PC;
   constexpr int NREG = regs_needed<N, T, P>();
   constexpr bool FIO = C_::FIO && P::S >= 2 && N / radix_at<P, false>(0) == T && TL::KST == NREG;
   constexpr bool FIN = FIO && !INIT, FOUT = FIO && !EXIT;
@@ -1100,7 +1102,7 @@ template <int N, int DIR, class C_ = Cfg<N>>
 __global__ __launch_bounds__(C_::TC * C_::C) void k_col_fft(float2* __restrict__ data,
                                                                      int nx, long long stride,
                                                                      const float2* __restrict__ twg) {
-  using P = typename C_::P;
+  using P = typename C_::PC;
   constexpr int T = C_::TC, C = C_::C;
   using L = TileLayout<N, C, C_::PADSH>;
   constexpr int NREG = regs_needed<N, T, P>();

@@ -115,7 +115,7 @@ def run(out_path):
 
 
 def choose(results):
-    """{n: dict(lpb, gtw, cw, wpe)} from the timing lines; defaults unless a
+    """{n: dict(lpb, gtw, cw, wpe, tw)} from the timing lines; defaults unless a
     variant is more than 2% faster"""
     by = {}
     for r in results:
@@ -131,17 +131,20 @@ def choose(results):
             if b["ms"] < 0.98 * d["ms"]:
                 k["lpb"], k["gtw"] = b["lpb"], b["gtw"]
         if cols:
-            d = next(r for r in cols if r["cw"] == 16 and r["wpe"] == 1)
+            d = next(r for r in cols if r["cw"] == 16 and r["wpe"] == 1 and r.get("tw", 0) == 0)
             b = min(cols, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
-                k["cw"], k["wpe"] = b["cw"], b["wpe"]
+                k["cw"], k["wpe"], k["tw"] = b["cw"], b["wpe"], b.get("tw", 0)
         plans = [r for r in rs if r["pass"] == "plan"]
-        if plans:
+        if plans:  # the row and the column transforms each take their fastest plan
             dname = ",".join(map(str, default_plan(n)))
             d = next((r for r in plans if r["plan"] == dname), None)
-            b = min(plans, key=lambda r: r["row_ms"] + r["col_ms"])
-            if d and b["plan"] != dname and b["row_ms"] + b["col_ms"] < 0.98 * (d["row_ms"] + d["col_ms"]):
-                k["plan"] = b["plan"]
+            br = min(plans, key=lambda r: r["row_ms"])
+            bc = min(plans, key=lambda r: r["col_ms"])
+            if d and br["plan"] != dname and br["row_ms"] < 0.98 * d["row_ms"]:
+                k["plan"] = br["plan"]
+            if d and bc["plan"] != k.get("plan", dname) and bc["col_ms"] < 0.98 * d["col_ms"]:
+                k["col_plan"] = bc["plan"]
         out[n] = k
     return out
 
@@ -154,12 +157,18 @@ def emit(results, path):
              "// jms_cfg.hpp: LPB (rows per row-pass block; 0 = default), GTW (row-pass",
              "// twiddles from the global table with the product tree), CW (columns per",
              "// wave-local column block), WPE (waves per SIMD asked of the register",
-             "// allocator for that kernel).  Included by jms_cfg.hpp after the plans.",
+             "// allocator for that kernel), TW (threads per column of that kernel; 0 =",
+             "// default).  Included by jms_cfg.hpp after the plans.",
              "#pragma once", "", "namespace jms {", "",
              "// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another",
              "template <int N>",
              "struct TunedPlan {",
              "  using type = typename PlanFor<N>::type;",
+             "};", "",
+             "// FFT plan of the column transforms: the row plan unless the tuner chose another",
+             "template <int N>",
+             "struct TunedColPlan {",
+             "  using type = typename TunedPlan<N>::type;",
              "};", "",
              "template <int N>",
              "struct Tuned {",
@@ -167,23 +176,28 @@ def emit(results, path):
              "  static constexpr bool GTW = false;",
              "  static constexpr int CW = 16;",
              "  static constexpr int WPE = 1;",
+             "  static constexpr int TW = 0;",
              "};", ""]
+    for n, k in ch.items():
+        if "col_plan" in k:
+            lines.append(f"template <> struct TunedColPlan<{n}> {{ using type = Radices<{k['col_plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
         if "plan" in k:
             lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
-        if not {"lpb", "gtw", "cw", "wpe"} & set(k):
+        if not {"lpb", "gtw", "cw", "wpe", "tw"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
         lines.append(f"  static constexpr bool GTW = {'true' if k.get('gtw') else 'false'};")
         lines.append(f"  static constexpr int CW = {k.get('cw', 16)};")
         lines.append(f"  static constexpr int WPE = {k.get('wpe', 1)};")
+        lines.append(f"  static constexpr int TW = {k.get('tw', 0)};")
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))
     print(f"wrote {path}: {sum(1 for k in ch.values() if k)} tuned lengths, "
-          f"{sum(1 for k in ch.values() if 'plan' in k)} with another plan")
+          f"{sum(1 for k in ch.values() if 'plan' in k or 'col_plan' in k)} with another plan")
 
 
 def main(argv):

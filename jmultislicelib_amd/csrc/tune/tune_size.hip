@@ -21,12 +21,14 @@ struct RowV : B {
   static constexpr bool GTW_ROW = G;
   static constexpr bool TWR_ROW = G;
 };
-// column variants (wave-local kernel): columns per block x waves per SIMD
-template <int CWV, int WPE>
+// column variants (wave-local kernel): columns per block x waves per SIMD x
+// threads per column (0: default)
+template <int CWV, int WPE, int TWV = 0>
 struct ColV : B {
   static constexpr int CW = CWV;
   static constexpr int CC = CWV;
   static constexpr int WPE_COL = WPE;
+  static constexpr int TW = TWV > 0 ? TWV : B::TW;
 };
 
 #define CK(x)                                                                  \
@@ -98,11 +100,11 @@ void row_variant(const Bufs& b, const char* name, int lpb, bool gtw, double byte
 }
 
 template <class CF>
-void col_variant(const Bufs& b, int cw, int wpe, double bytes) {
+void col_variant(const Bufs& b, int cw, int wpe, double bytes, int tw = 0) {
   float best = 1e30f;
   for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
-  std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n", N, cw, wpe, best,
-              bytes / (best * 1e-3) / 1e9);
+  std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"tw\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n",
+              N, cw, wpe, tw, best, bytes / (best * 1e-3) / 1e9);
 }
 
 // FFT plan candidates (a file written by tune.py: JMS_TUNE_PLANS(X) lists
@@ -113,7 +115,7 @@ void col_variant(const Bufs& b, int cw, int wpe, double bytes) {
 
 template <class PL>
 void plan_variant(const Bufs& b, const char* name, double bytes) {
-  using C = CfgP<N, PL>;
+  using C = CfgP<N, PL, PL>;
   float br = 1e30f, bc = 1e30f;
   for (int r = 0; r < 3; ++r) {
     br = std::fmin(br, time_row<C>(b, 10));
@@ -162,6 +164,20 @@ int main() {
     if constexpr (B::TW % 8 == 0) {
       col_variant<ColV<8, 1>>(b, 8, 1, bytes);
       col_variant<ColV<8, 8>>(b, 8, 8, bytes);
+    }
+    // 32 threads per column (two columns per wavefront): half the block, so
+    // two 16-column blocks can share a CU where the LDS tile allows it
+    if constexpr (B::TW == 64) {
+      col_variant<ColV<16, 1, 32>>(b, 16, 1, bytes, 32);
+      col_variant<ColV<16, 4, 32>>(b, 16, 4, bytes, 32);
+      col_variant<ColV<12, 1, 32>>(b, 12, 1, bytes, 32);
+      col_variant<ColV<12, 1>>(b, 12, 1, bytes);
+    }
+    // 128 threads per column (two wavefronts, block barriers in the
+    // transform): every radix stage in one round, far fewer registers
+    if constexpr (B::T > 64 && B::T <= 128) {
+      col_variant<ColV<8, 1, 128>>(b, 8, 1, bytes, 128);
+      col_variant<ColV<8, 8, 128>>(b, 8, 8, bytes, 128);
     }
   }
 #ifdef JMS_PLANS_FILE
