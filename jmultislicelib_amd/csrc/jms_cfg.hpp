@@ -57,6 +57,9 @@ struct CfgP {
   // columns per block of the column kernel in use (partial-sum tiles)
   static constexpr int CC = XL ? 16 : (WL ? CW : C);
   static constexpr int PADSH = 3;
+  // LDS padding of the wave-local column kernel's tile (a shift >= 20: no
+  // padding, every exchange address a constant offset from one base)
+  static constexpr int PADSH_COL = Tuned<N>::PADC > 0 ? Tuned<N>::PADC : PADSH;
   // minimum waves per SIMD requested from the register allocator for the
   // wave-local column kernel (1 = no constraint; jms_tuned.h).  Eight waves
   // let two 1024-thread blocks share a CU, so one block's FFTs overlap the

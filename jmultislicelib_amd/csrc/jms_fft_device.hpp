@@ -407,9 +407,14 @@ constexpr int plan_product() {
 template <int SW, int PADSH>
 struct LdsPos {
   static constexpr bool AFFINE = (SW == 0);
+  // PADSH >= 20: no padding at all (every element at its own index, so any
+  // stride gives constant offsets from one base)
+  static constexpr bool FLAT = (SW == 0) && (PADSH >= 20);
   static constexpr int PAD = PADSH;
   __device__ static __forceinline__ int pos(int e) {
-    if constexpr (SW == 0) {
+    if constexpr (FLAT) {
+      return e;
+    } else if constexpr (SW == 0) {
       return e + (e >> PADSH);
     } else {
       const int r = e >> 5;
@@ -417,8 +422,8 @@ struct LdsPos {
     }
   }
   // slots a line of n elements occupies
-  static constexpr int span(int n) { return SW == 0 ? n + (n >> PADSH) + 1 : ((n + 31) / 32) * 32; }
-  static constexpr int step(int d) { return d + (d >> PADSH); }
+  static constexpr int span(int n) { return FLAT ? n : (SW == 0 ? n + (n >> PADSH) + 1 : ((n + 31) / 32) * 32); }
+  static constexpr int step(int d) { return FLAT ? d : d + (d >> PADSH); }
 };
 
 template <int N, int PADSH, int SW = 0>
@@ -440,7 +445,7 @@ struct TileLayout : LdsPos<0, PADSH> {  // LINES lines interleaved (line fastest
 // butterfly), by a shared base when the layout allows it
 template <class Layout, int NBF, int Q>
 __device__ __forceinline__ int lds_slot(int line, int b, int a0) {
-  if constexpr (Layout::AFFINE && NBF % (1 << Layout::PAD) == 0) return a0 + Layout::step(Q * NBF);
+  if constexpr (Layout::AFFINE && (Layout::FLAT || NBF % (1 << Layout::PAD) == 0)) return a0 + Layout::step(Q * NBF);
   else return Layout::idx(line, b + Q * NBF);
 }
 
@@ -525,7 +530,8 @@ struct LineFFT {
           const int o0 = (b % s) + (b / s) * s * R;
           // o0 is on a padding-period boundary when s == 1 and R is a multiple
           // of the period; s*k is a multiple of it when s is
-          constexpr bool CONST_OFF = Layout::AFFINE && ((s % (1 << Layout::PAD) == 0) || (s == 1 && R % (1 << Layout::PAD) == 0));
+          constexpr bool CONST_OFF = Layout::AFFINE && (Layout::FLAT || (s % (1 << Layout::PAD) == 0) ||
+                                                        (s == 1 && R % (1 << Layout::PAD) == 0));
           const int a0 = Layout::idx(line, o0);
           static_for<0, R>([&](auto K) {
             constexpr int k = decltype(K)::value;

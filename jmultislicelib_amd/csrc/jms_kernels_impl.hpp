@@ -475,7 +475,7 @@ struct ColTile {
   static constexpr int T = C_::TW, C = C_::CW, BS = T * C;
   static constexpr int KST = (N * C + BS - 1) / BS;
   static constexpr bool STFULL = (N * C) % BS == 0;
-  using L = ColLayout<N, C_::PADSH, C_::SW>;
+  using L = ColLayout<N, C_::PADSH_COL, C_::SW>;
   // global -> registers (columns past the grid edge read column 0 of the
   // tile and are discarded)
   static __device__ __forceinline__ void load(float2 (&st)[KST], const float2* src, int nx, int ncol,
@@ -527,7 +527,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
                                          float* s_red, int c, int t, int x, bool valid) {
   using P = typename C_::PC;
   constexpr int T = C_::TW;
-  using L = ColLayout<N, C_::PADSH, C_::SW>;
+  using L = ColLayout<N, C_::PADSH_COL, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   using M0 = IOMap<N, T, P, false>;
   using M1 = IOMap<N, T, P, true>;
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   constexpr int T = C_::TW, C = C_::CW, BS = T * C;
   static_assert(64 % T == 0 || T % 64 == 0, "a column is part of one wavefront or whole wavefronts");
   static_assert(BS % 64 == 0, "block must be whole wavefronts");
-  using L = ColLayout<N, C_::PADSH, C_::SW>;
+  using L = ColLayout<N, C_::PADSH_COL, C_::SW>;
   using TL = ColTile<N, C_>;
   __shared__ float2 s_tw[N];
   __shared__ float2 s_buf[L::words(C)];

@@ -5,7 +5,8 @@
 // twiddles from the global table with the product tree), CW (columns per
 // wave-local column block), WPE (waves per SIMD asked of the register
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
-// default).  Included by jms_cfg.hpp after the plans.
+// default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
+// none).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
@@ -29,6 +30,7 @@ struct Tuned {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 
 template <> struct TunedColPlan<360> { using type = Radices<9, 5, 8>; };
@@ -44,6 +46,7 @@ template <> struct Tuned<16> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<48> {
   static constexpr int LPB = 64;
@@ -51,6 +54,7 @@ template <> struct Tuned<48> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<64> {
   static constexpr int LPB = 64;
@@ -58,6 +62,7 @@ template <> struct Tuned<64> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<96> {
   static constexpr int LPB = 42;
@@ -65,6 +70,7 @@ template <> struct Tuned<96> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<128> {
   static constexpr int LPB = 16;
@@ -72,6 +78,7 @@ template <> struct Tuned<128> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<160> {
   static constexpr int LPB = 16;
@@ -79,6 +86,7 @@ template <> struct Tuned<160> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<192> {
   static constexpr int LPB = 8;
@@ -86,6 +94,7 @@ template <> struct Tuned<192> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<256> {
   static constexpr int LPB = 8;
@@ -93,6 +102,7 @@ template <> struct Tuned<256> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<320> {
   static constexpr int LPB = 8;
@@ -100,13 +110,15 @@ template <> struct Tuned<320> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
   static constexpr bool GTW = false;
-  static constexpr int CW = 8;
+  static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 24;
 };
 template <> struct Tuned<384> {
   static constexpr int LPB = 8;
@@ -114,6 +126,7 @@ template <> struct Tuned<384> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<480> {
   static constexpr int LPB = 6;
@@ -121,6 +134,7 @@ template <> struct Tuned<480> {
   static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 24;
 };
 template <> struct Tuned<512> {
   static constexpr int LPB = 4;
@@ -128,6 +142,7 @@ template <> struct Tuned<512> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 24;
 };
 template <> struct Tuned<625> {
   static constexpr int LPB = 4;
@@ -135,6 +150,7 @@ template <> struct Tuned<625> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -142,6 +158,7 @@ template <> struct Tuned<640> {
   static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<720> {
   static constexpr int LPB = 2;
@@ -149,6 +166,7 @@ template <> struct Tuned<720> {
   static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
@@ -156,6 +174,7 @@ template <> struct Tuned<768> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
@@ -163,6 +182,7 @@ template <> struct Tuned<1024> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
@@ -170,6 +190,7 @@ template <> struct Tuned<2048> {
   static constexpr int CW = 16;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
+  static constexpr int PADC = 0;
 };
 
 }  // namespace jms

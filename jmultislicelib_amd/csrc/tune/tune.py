@@ -131,10 +131,11 @@ def choose(results):
             if b["ms"] < 0.98 * d["ms"]:
                 k["lpb"], k["gtw"] = b["lpb"], b["gtw"]
         if cols:
-            d = next(r for r in cols if r["cw"] == 16 and r["wpe"] == 1 and r.get("tw", 0) == 0)
+            cols = [r for r in cols if r.get("twr", 1) == 1]  # per-leg twiddle reads: timed, not selectable
+            d = cols[0]  # the first variant timed: the current configuration
             b = min(cols, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
-                k["cw"], k["wpe"], k["tw"] = b["cw"], b["wpe"], b.get("tw", 0)
+                k["cw"], k["wpe"], k["tw"], k["pad"] = b["cw"], b["wpe"], b.get("tw", 0), b.get("pad", 3)
         plans = [r for r in rs if r["pass"] == "plan"]
         if plans:  # the row and the column transforms each take their fastest plan
             dname = ",".join(map(str, default_plan(n)))
@@ -158,7 +159,8 @@ def emit(results, path):
              "// twiddles from the global table with the product tree), CW (columns per",
              "// wave-local column block), WPE (waves per SIMD asked of the register",
              "// allocator for that kernel), TW (threads per column of that kernel; 0 =",
-             "// default).  Included by jms_cfg.hpp after the plans.",
+             "// default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20",
+             "// none).  Included by jms_cfg.hpp after the plans.",
              "#pragma once", "", "namespace jms {", "",
              "// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another",
              "template <int N>",
@@ -177,6 +179,7 @@ def emit(results, path):
              "  static constexpr int CW = 16;",
              "  static constexpr int WPE = 1;",
              "  static constexpr int TW = 0;",
+             "  static constexpr int PADC = 0;",
              "};", ""]
     for n, k in ch.items():
         if "col_plan" in k:
@@ -185,7 +188,7 @@ def emit(results, path):
         if "plan" in k:
             lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
-        if not {"lpb", "gtw", "cw", "wpe", "tw"} & set(k):
+        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
@@ -193,6 +196,7 @@ def emit(results, path):
         lines.append(f"  static constexpr int CW = {k.get('cw', 16)};")
         lines.append(f"  static constexpr int WPE = {k.get('wpe', 1)};")
         lines.append(f"  static constexpr int TW = {k.get('tw', 0)};")
+        lines.append(f"  static constexpr int PADC = {0 if k.get('pad', 3) == 3 else k['pad']};")
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))

@@ -23,12 +23,14 @@ struct RowV : B {
 };
 // column variants (wave-local kernel): columns per block x waves per SIMD x
 // threads per column (0: default)
-template <int CWV, int WPE, int TWV = 0>
+template <int CWV, int WPE, int TWV = 0, bool TWRV = B::TWR, int PADV = B::PADSH_COL>
 struct ColV : B {
   static constexpr int CW = CWV;
   static constexpr int CC = CWV;
   static constexpr int WPE_COL = WPE;
   static constexpr int TW = TWV > 0 ? TWV : B::TW;
+  static constexpr bool TWR = TWRV;
+  static constexpr int PADSH_COL = PADV;
 };
 
 #define CK(x)                                                                  \
@@ -103,8 +105,9 @@ template <class CF>
 void col_variant(const Bufs& b, int cw, int wpe, double bytes, int tw = 0) {
   float best = 1e30f;
   for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
-  std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"tw\": %d, \"ms\": %.5f, \"gbs\": %.1f}\n",
-              N, cw, wpe, tw, best, bytes / (best * 1e-3) / 1e9);
+  std::printf("{\"n\": %d, \"pass\": \"col\", \"cw\": %d, \"wpe\": %d, \"tw\": %d, \"twr\": %d, \"pad\": %d, "
+              "\"ms\": %.5f, \"gbs\": %.1f}\n", N, cw, wpe, tw, (int)CF::TWR, CF::PADSH_COL, best,
+              bytes / (best * 1e-3) / 1e9);
 }
 
 // FFT plan candidates (a file written by tune.py: JMS_TUNE_PLANS(X) lists
@@ -173,6 +176,13 @@ int main() {
       col_variant<ColV<12, 1, 32>>(b, 12, 1, bytes, 32);
       col_variant<ColV<12, 1>>(b, 12, 1, bytes);
     }
+    // one table read per twiddle leg instead of the product tree (fewer
+    // VALU instructions, more LDS reads)
+    col_variant<ColV<16, 1, 0, false>>(b, 16, 1, bytes);
+    if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, false>>(b, 8, 1, bytes);
+    // no LDS padding: every exchange address a constant offset from one base
+    col_variant<ColV<16, 1, 0, B::TWR, 24>>(b, 16, 1, bytes);
+    if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 24>>(b, 8, 1, bytes);
     // 128 threads per column (two wavefronts, block barriers in the
     // transform): every radix stage in one round, far fewer registers
     if constexpr (B::T > 64 && B::T <= 128) {
