@@ -60,6 +60,8 @@ struct CfgP {
   // LDS padding of the wave-local column kernel's tile (a shift >= 20: no
   // padding, every exchange address a constant offset from one base)
   static constexpr int PADSH_COL = Tuned<N>::PADC > 0 ? Tuned<N>::PADC : PADSH;
+  // the same for the row pass's line exchanges
+  static constexpr int PADSH_ROW = Tuned<N>::PADR > 0 ? Tuned<N>::PADR : PADSH;
   // minimum waves per SIMD requested from the register allocator for the
   // wave-local column kernel (1 = no constraint; jms_tuned.h).  Eight waves
   // let two 1024-thread blocks share a CU, so one block's FFTs overlap the

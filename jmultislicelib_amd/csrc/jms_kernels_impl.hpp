@@ -123,7 +123,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   static_assert(!GRP || MODE == 0, "grouped row passes carry no real-space readout");
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
-  using L = LineLayout<N, C_::PADSH, C_::SW>;
+  using L = LineLayout<N, C_::PADSH_ROW, C_::SW>;
   constexpr int NREG = regs_needed<N, T, P>();
   // twiddles from an LDS copy of the table, or (GTW_ROW) read through the
   // caches from the global table
@@ -860,9 +860,13 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   constexpr int CT = 16;      // columns per block
   __shared__ float2 s_x[16 * 64 * CT];  // one round: [k2 % 16][n1][column]
   __shared__ float2 s_tw[N];
+  // W_64^(j*c) at [c][j] (j = 0..3, c = 0..15): the four lanes of a quad
+  // read four adjacent entries (in s_tw they would share one bank)
+  __shared__ float2 s_w64[64];
   __shared__ float s_red[16 * kMaxDet];
   const int tid = threadIdx.x;
   for (int e = tid; e < N; e += 1024) s_tw[e] = a.twg[e];
+  if (tid < 64) s_w64[tid] = a.twg[((tid & 3) * (tid >> 2) * M) % N];
   int bx, by;
   block_xy<C_::XCD_COL>(bx, by);
   const long long w = by;
@@ -882,6 +886,14 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   const float sg1 = j0 ? -1.f : 1.f, sg2 = j1 ? -1.f : 1.f;
   const bool validb = cb < ncol;
   const int xb = x0 + (validb ? cb : 0);
+  // exchange buffer [k2 % 16][n1][column], columns XOR-swizzled: rows with
+  // bit 1 set hold column c at c ^ 8.  The B-layout accesses (rows j + 4m of
+  // a quad's four lanes, 8 columns per half-wavefront) then fall on 64
+  // distinct banks instead of two lanes per bank; the A-layout accesses
+  // stay a permutation inside one row
+  static_assert(CT == 16, "swizzle assumes 16 columns");
+  const int swa = ca ^ (((n1 >> 1) & 1) << 3);
+  const int swb = cb ^ (j1 << 3);
   // slot (r, c) of the B layout <-> row ky = 16r + M*c (uniform part) + kyb
   const int kyb = wv + 16 * M * dq;
   auto kyof = [&](int r, int c) { return 16 * r + M * c + kyb; };
@@ -938,19 +950,19 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       if constexpr (r > 0) __syncthreads();  // the previous round's reads are done
       static_for<0, 16>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        s_x[(k * 64 + n1) * CT + ca] = u[k];
+        s_x[(k * 64 + n1) * CT + swa] = u[k];
       });
       __syncthreads();
       static_for<0, 16>([&](auto K) {
         constexpr int m = decltype(K)::value;
-        u[m] = s_x[(wv * 64 + j + 4 * m) * CT + cb];
+        u[m] = s_x[(wv * 64 + j + 4 * m) * CT + swb];
       });
       DFT<16, -1>::run(u);
       __builtin_amdgcn_sched_barrier(0);
-      const int jt = opaque(j) * M;
+      const float2* w64 = s_w64 + opaque(j);
       static_for<1, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
-        u[c] = cmul(u[c], s_tw[jt * c]);  // W_64^(j*c)
+        u[c] = cmul(u[c], w64[4 * c]);  // W_64^(j*c)
       });
       static_for<0, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
@@ -1021,22 +1033,22 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
         if (j0 & j1) z = mul_si<+1>(z);
         u[c] = quad_bfly<2>(z, sg2);
       });
-      const int jt = opaque(j) * M;
+      const float2* w64 = s_w64 + opaque(j);
       static_for<1, 16>([&](auto K) {
         constexpr int c = decltype(K)::value;
-        u[c] = cmulconj(u[c], s_tw[jt * c]);
+        u[c] = cmulconj(u[c], w64[4 * c]);
       });
       DFT<16, +1>::run(u);
       __builtin_amdgcn_sched_barrier(0);
       __syncthreads();  // earlier users of the exchange buffer are done
       static_for<0, 16>([&](auto K) {
         constexpr int m = decltype(K)::value;
-        s_x[(wv * 64 + j + 4 * m) * CT + cb] = u[m];
+        s_x[(wv * 64 + j + 4 * m) * CT + swb] = u[m];
       });
       __syncthreads();
       static_for<0, 16>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        u[k] = s_x[(k * 64 + n1) * CT + ca];
+        u[k] = s_x[(k * 64 + n1) * CT + swa];
       });
     });
     __builtin_amdgcn_sched_barrier(0);

@@ -6,7 +6,7 @@
 // wave-local column block), WPE (waves per SIMD asked of the register
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
 // default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
-// none).  Included by jms_cfg.hpp after the plans.
+// none), PADR (the same for the row pass).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
@@ -31,6 +31,7 @@ struct Tuned {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 
 template <> struct TunedColPlan<360> { using type = Radices<9, 5, 8>; };
@@ -47,6 +48,7 @@ template <> struct Tuned<16> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<48> {
   static constexpr int LPB = 64;
@@ -55,6 +57,7 @@ template <> struct Tuned<48> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<64> {
   static constexpr int LPB = 64;
@@ -63,6 +66,7 @@ template <> struct Tuned<64> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<96> {
   static constexpr int LPB = 42;
@@ -71,6 +75,7 @@ template <> struct Tuned<96> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<128> {
   static constexpr int LPB = 16;
@@ -79,6 +84,7 @@ template <> struct Tuned<128> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<160> {
   static constexpr int LPB = 16;
@@ -87,6 +93,7 @@ template <> struct Tuned<160> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<192> {
   static constexpr int LPB = 8;
@@ -95,6 +102,7 @@ template <> struct Tuned<192> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<256> {
   static constexpr int LPB = 8;
@@ -103,6 +111,7 @@ template <> struct Tuned<256> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<320> {
   static constexpr int LPB = 8;
@@ -111,6 +120,7 @@ template <> struct Tuned<320> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
@@ -119,6 +129,7 @@ template <> struct Tuned<360> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
+  static constexpr int PADR = 24;
 };
 template <> struct Tuned<384> {
   static constexpr int LPB = 8;
@@ -127,6 +138,7 @@ template <> struct Tuned<384> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<480> {
   static constexpr int LPB = 6;
@@ -135,6 +147,7 @@ template <> struct Tuned<480> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
+  static constexpr int PADR = 24;
 };
 template <> struct Tuned<512> {
   static constexpr int LPB = 4;
@@ -143,6 +156,7 @@ template <> struct Tuned<512> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<625> {
   static constexpr int LPB = 4;
@@ -151,6 +165,7 @@ template <> struct Tuned<625> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -159,6 +174,7 @@ template <> struct Tuned<640> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 24;
 };
 template <> struct Tuned<720> {
   static constexpr int LPB = 2;
@@ -167,6 +183,7 @@ template <> struct Tuned<720> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 24;
 };
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
@@ -175,6 +192,7 @@ template <> struct Tuned<768> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
@@ -183,6 +201,7 @@ template <> struct Tuned<1024> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
@@ -191,6 +210,7 @@ template <> struct Tuned<2048> {
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
+  static constexpr int PADR = 0;
 };
 
 }  // namespace jms

@@ -122,7 +122,10 @@ def choose(results):
         by.setdefault(r["n"], []).append(r)
     out = {}
     for n, rs in sorted(by.items()):
-        rows = [r for r in rs if r["pass"] == "row"]
+        tuned = {r["variant"]: r for r in rs if r["pass"] == "row" and r["variant"].startswith("tuned")}
+        if "tuned" in tuned and "tuned_flat" in tuned and tuned["tuned_flat"]["ms"] < 0.98 * tuned["tuned"]["ms"]:
+            k["padr"] = 24
+        rows = [r for r in rs if r["pass"] == "row" and not r["variant"].startswith("tuned")]
         cols = [r for r in rs if r["pass"] == "col"]
         k = {}
         if rows:
@@ -160,7 +163,8 @@ def emit(results, path):
              "// wave-local column block), WPE (waves per SIMD asked of the register",
              "// allocator for that kernel), TW (threads per column of that kernel; 0 =",
              "// default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20",
-             "// none).  Included by jms_cfg.hpp after the plans.",
+             "// none), PADR (the same for the row pass).  Included by jms_cfg.hpp after",
+             "// the plans.",
              "#pragma once", "", "namespace jms {", "",
              "// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another",
              "template <int N>",
@@ -180,6 +184,7 @@ def emit(results, path):
              "  static constexpr int WPE = 1;",
              "  static constexpr int TW = 0;",
              "  static constexpr int PADC = 0;",
+             "  static constexpr int PADR = 0;",
              "};", ""]
     for n, k in ch.items():
         if "col_plan" in k:
@@ -188,7 +193,7 @@ def emit(results, path):
         if "plan" in k:
             lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
-        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad"} & set(k):
+        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad", "padr"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
@@ -197,6 +202,7 @@ def emit(results, path):
         lines.append(f"  static constexpr int WPE = {k.get('wpe', 1)};")
         lines.append(f"  static constexpr int TW = {k.get('tw', 0)};")
         lines.append(f"  static constexpr int PADC = {0 if k.get('pad', 3) == 3 else k['pad']};")
+        lines.append(f"  static constexpr int PADR = {k.get('padr', 0)};")
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))

@@ -21,6 +21,12 @@ struct RowV : B {
   static constexpr bool GTW_ROW = G;
   static constexpr bool TWR_ROW = G;
 };
+// the tuned row shape with a given LDS padding shift (3: one slot per 8
+// elements; >= 20: none)
+template <int PADV>
+struct RowP : B {
+  static constexpr int PADSH_ROW = PADV;
+};
 // column variants (wave-local kernel): columns per block x waves per SIMD x
 // threads per column (0: default)
 template <int CWV, int WPE, int TWV = 0, bool TWRV = B::TWR, int PADV = B::PADSH_COL>
@@ -151,6 +157,8 @@ int main() {
   constexpr int L0 = (256 / B::T) > 0 ? (256 / B::T) : 1;  // default rows per block
   constexpr int L2 = (B::T * 2 * L0 <= 1024) ? 2 * L0 : L0;
   constexpr int LH = L0 > 1 ? L0 / 2 : 1;
+  row_variant<RowP<3>>(b, "tuned", B::LPB, B::GTW_ROW, bytes);
+  row_variant<RowP<24>>(b, "tuned_flat", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowV<L0, false>>(b, "default", L0, false, bytes);
   row_variant<RowV<L0, true>>(b, "default", L0, true, bytes);
   if (L2 != L0) {
