@@ -159,6 +159,9 @@ int main() {
   constexpr int LH = L0 > 1 ? L0 / 2 : 1;
   row_variant<RowP<3>>(b, "tuned", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowP<24>>(b, "tuned_flat", B::LPB, B::GTW_ROW, bytes);
+  row_variant<RowP<2>>(b, "tuned_pad2", B::LPB, B::GTW_ROW, bytes);
+  row_variant<RowP<4>>(b, "tuned_pad4", B::LPB, B::GTW_ROW, bytes);
+  row_variant<RowP<5>>(b, "tuned_pad5", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowV<L0, false>>(b, "default", L0, false, bytes);
   row_variant<RowV<L0, true>>(b, "default", L0, true, bytes);
   if (L2 != L0) {
