@@ -127,7 +127,8 @@ int jms_hip_sync(jms_hip_engine* e);
  * loops).  Runs the slice loop from plane islc0 on nwave = npos * npass waves
  * of the set-up specimen, as one batch of jms_hip_run_staged does, with every
  * buffer in device memory that the caller owns: waves (nwave * nx * ny
- * complex64, working and exit waves), ppar (float4 per wave: dx, dy [nm],
+ * complex64 of working storage, laid out as the staged batches keep their
+ * waves), ppar (float4 per wave: dx, dy [nm],
  * dz * lambda / 2, 0), var (nobjslc ints per wave), weights (npass), res
  * (npos * ndetslc * ndet floats, overwritten).  Enqueued on the engine's
  * stream; jms_hip_sync waits.  The readout partial sums use a fixed 16 MiB

@@ -62,6 +62,10 @@ struct CfgP {
   static constexpr int PADSH_COL = Tuned<N>::PADC > 0 ? Tuned<N>::PADC : PADSH;
   // the same for the row pass's line exchanges
   static constexpr int PADSH_ROW = Tuned<N>::PADR > 0 ? Tuned<N>::PADR : PADSH;
+  // waves of staged batches stored in panels of 16 columns (see panel_off):
+  // contiguous column tiles, rows in 128-byte segments; where the tuner found
+  // the pair of passes faster that way
+  static constexpr bool PANEL = Tuned<N>::PANEL && (N % 16 == 0);
   // minimum waves per SIMD requested from the register allocator for the
   // wave-local column kernel (1 = no constraint; jms_tuned.h).  Eight waves
   // let two 1024-thread blocks share a CU, so one block's FFTs overlap the

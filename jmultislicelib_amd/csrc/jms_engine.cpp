@@ -105,6 +105,8 @@ struct jms_hip_engine {
   std::array<const void*, 16> graph_bufs{};
   // this device's scratch of the allocation-free batch entry (resolved at create)
   float* scratch = nullptr;
+  // staged batches keep their waves in 16-column panels (set with the grid)
+  bool panel = false;
   // timing
   bool timing = false;
   Timer tm[4];  // row pass, column pass, combine, graph replay
@@ -224,7 +226,11 @@ struct BatchChans {
 // (npos results) when npass > 0; `ch` are the single-wave channels.
 int enqueue_batch(jms_hip_engine* e, void* waves, float* partial, int nwave, int islc0, const void* src,
                   long long src_stride, const void* ppar, const int* var, int npos, int npass,
-                  const float* weights, float* res, int accumulate, const BatchChans& ch, bool timed) {
+                  const float* weights, float* res, int accumulate, const BatchChans& ch, bool timed,
+                  bool panel = false) {
+  // panel waves (staged batches): integrated readouts only, the waves are internal
+  if (panel && (ch.real_space() || ch.dif || ch.wff || ch.dif_avg || ch.wff_avg))
+    return e->fail(135, "panel waves carry no readout channels");
   const int S = e->nobjslc;
   const int ntc = e->ntiles_col();
   const long long part_w = (long long)e->ndetslc * e->ndet * ntc;
@@ -258,6 +264,7 @@ int enqueue_batch(jms_hip_engine* e, void* waves, float* partial, int nwave, int
   rc.cwgt = ch.cwgt;
   rc.chan_wstride = ca.chan_wstride;
   ca.twy = e->twy.p;
+  ca.panel = panel;
   hipError_t err;
   auto plane_setup = [&](int plane) {
     const int id = (plane >= 0 && plane <= S) ? e->det_objslc[plane] : -1;
@@ -288,7 +295,7 @@ int enqueue_batch(jms_hip_engine* e, void* waves, float* partial, int nwave, int
       TimedLaunch tl(e, 0, timed);
       err = jms::launch_row_pass(e->nx, e->ny, nwave, (float2*)waves, e->n,
                                  e->pgr.as<float2>(), e->slcoffset[js], (long long)e->npgx * e->npgy,
-                                 var, S, j, e->twx.as<float2>(), rc, mode, e->st);
+                                 var, S, j, e->twx.as<float2>(), rc, mode, panel, e->st);
       if (err != hipSuccess) return e->fail(131, "row pass", err);
     }
     ca.pro = e->pro.as<float2>() + (long long)e->proidx[js] * e->n;
@@ -326,7 +333,7 @@ int enqueue_exit_real_readout(jms_hip_engine* e, int nwave, const BatchChans& ch
   hipError_t err = jms::launch_col_fft(e->nx, e->ny, nwave, e->waves.as<float2>(), +1, e->twy.as<float2>(), e->st);
   if (err == hipSuccess)
     err = jms::launch_row_pass(e->nx, e->ny, nwave, e->waves.as<float2>(), e->n, e->pgr.as<float2>(), 0, 0,
-                               nullptr, 0, 0, e->twx.as<float2>(), rc, 2, e->st);
+                               nullptr, 0, 0, e->twx.as<float2>(), rc, 2, false, e->st);
   if (err != hipSuccess) return e->fail(134, "exit-plane real-space readout", err);
   return 0;
 }
@@ -414,6 +421,7 @@ int jms_hip_set_grid(jms_hip_engine* e, int nx, int ny, float a0x, float a0y, fl
   e->drop_graphs();
   e->nx = nx; e->ny = ny; e->n = (long long)nx * ny;
   e->a0x = a0x; e->a0y = a0y; e->wl = wl;
+  e->panel = jms::panel_ok(nx, ny);
   std::vector<float> tw;
   make_twiddles(nx, tw);
   hipError_t err = e->twx.ensure(tw.size() * sizeof(float));
@@ -894,7 +902,8 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
         if (err != hipSuccess) return e->fail(121, "begin capture", err);
         rc = enqueue_batch(e, e->waves.p, e->partial.as<float>(), nw, islc0, e->wav0.p, 0, e->cur_ppar.p,
                            e->cur_var.as<int>(), np,
-                           npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, BatchChans(), false);
+                           npass, e->s_weights.as<float>(), e->cur_res.as<float>(), 0, BatchChans(), false,
+                           e->panel);
         hipError_t err2 = hipStreamEndCapture(e->st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         if (err2 != hipSuccess) return e->fail(122, "end capture", err2);
@@ -914,7 +923,7 @@ int jms_hip_run_staged(jms_hip_engine* e, int pos0, int npos, int islc0) {
     } else {
       rc = enqueue_batch(e, e->waves.p, e->partial.as<float>(), nw, islc0, e->wav0.p, 0, ppar, var, np, npass,
                          e->s_weights.as<float>(), res,
-                         0, BatchChans(), e->timing);
+                         0, BatchChans(), e->timing, e->panel);
       if (rc) return rc;
     }
   }
@@ -943,7 +952,7 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
   islc0 = std::max(0, std::min(islc0, e->nobjslc));
   (void)hipSetDevice(e->device);
   return enqueue_batch(e, waves, e->scratch, nwave, islc0, e->wav0.p, 0, ppar, var, npos, npass, weights, res, 0,
-                       BatchChans(), false);
+                       BatchChans(), false, e->panel);
 }
 
 // ---------------------------------------------------------------------------
@@ -1191,7 +1200,7 @@ int jms_hip_group_run(jms_hip_group* g) {
       ga.step = plane;
       TimedLaunch tl(lead, 1, lead->timing);
       return jms::launch_col_pass_grp(g->nx, g->ny, w1 - w0, init, exit_, group_detects(g, rd, plane, w0, w1), ntc,
-                                      lead->twy.as<float2>(), ga, st);
+                                      lead->twy.as<float2>(), ga, lead->panel, st);
     };
     err = col(0, true, 0, rd.active[0], false);
     if (err == hipSuccess) err = col(0, true, rd.active[0], rd.nwave, true);
@@ -1200,7 +1209,7 @@ int jms_hip_group_run(jms_hip_group* g) {
       ga.step = s;
       {
         TimedLaunch tl(lead, 0, lead->timing);
-        err = jms::launch_row_pass_grp(g->nx, g->ny, rd.active[s], n, n, lead->twx.as<float2>(), ga, st);
+        err = jms::launch_row_pass_grp(g->nx, g->ny, rd.active[s], n, n, lead->twx.as<float2>(), ga, lead->panel, st);
       }
       if (err == hipSuccess) err = col(s + 1, false, 0, rd.active[s + 1], false);
       if (err == hipSuccess) err = col(s + 1, false, rd.active[s + 1], rd.active[s], true);

@@ -35,6 +35,7 @@ struct ColArgsHost {
   const void* pro_t = nullptr;   // propagator of this slice (float2)
   const float* detw_t = nullptr; // [ndet][nx][ny] detector weights
   const float* cppl_t = nullptr; // constant phase plate or nullptr
+  bool panel = false;            // the waves are in 16-column panels (panel_ok)
 };
 
 // real-space readout channels of one detection plane for the row pass
@@ -121,13 +122,13 @@ struct GrpArgs {
 bool grp_supported(int nx, int ny);
 // row pass of step ga.step over nwave waves of the map (from ga.w0)
 hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride, long long npg,
-                               const float2* twx, const GrpArgs& ga, hipStream_t st);
+                               const float2* twx, const GrpArgs& ga, bool panel, hipStream_t st);
 // column pass producing plane ga.step (propagator of step ga.step - 1 unless
 // init) over nwave waves of the map; detect: some job of the launch reads
 // plane ga.step out (each wave checks its own job's plane table)
 hipError_t launch_col_pass_grp(int nx, int ny, int nwave, bool init, bool exit_, bool detect,
                                long long part_dstride, const float2* twy, const GrpArgs& ga,
-                               hipStream_t st);
+                               bool panel, hipStream_t st);
 
 constexpr int kMaxDetPerSweep = 8;
 
@@ -136,11 +137,16 @@ int row_tiles(int nx, int ny);
 int col_tiles(int ny, int nx);
 
 // mode 0: slice; 1: slice with the real-space readout of `rc` first;
-// 2: inverse x-transform and readout only (exit plane, after a column IFFT)
+// 2: inverse x-transform and readout only (exit plane, after a column IFFT).
+// panel: the waves are in 16-column panels (mode 0 only; see panel_ok)
 hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
                            const float2* pgr, long long slc_off, long long npg, const int* var_tab,
                            int var_ld, int jslc, const float2* twx, const RowChanHost& rc, int mode,
-                           hipStream_t st);
+                           bool panel, hipStream_t st);
+// whether the staged batches of an nx x ny grid keep their waves in panels of
+// 16 columns: element (y, x) at (x / 16) * 16 ny + 16 y + x % 16 (column
+// tiles contiguous); chosen per line length where the passes run faster so
+bool panel_ok(int nx, int ny);
 // column transforms only (all columns of nbatch grids)
 hipError_t launch_col_fft(int nx, int ny, int nbatch, float2* data, int dir, const float2* twy,
                           hipStream_t st);

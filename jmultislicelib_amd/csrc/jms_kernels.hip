@@ -16,7 +16,7 @@ template <int N>
 hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
                              const float2* pgr, long long slc_off, long long npg,
                              const int* var_tab, int var_ld, int jslc, const float2* twx,
-                             const RowChanHost& rc, int mode, hipStream_t st);
+                             const RowChanHost& rc, int mode, bool panel, hipStream_t st);
 template <int N>
 hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost& h,
                              hipStream_t st);
@@ -28,10 +28,10 @@ hipError_t launch_col_fft_n(int nx, int nbatch, float2* data, long long stride, 
                             const float2* twy, hipStream_t st);
 template <int N>
 hipError_t launch_row_pass_grp_n(int ny, int nwave, long long wave_stride, long long npg, const float2* twx,
-                                 const GrpArgs& ga, hipStream_t st);
+                                 const GrpArgs& ga, bool panel, hipStream_t st);
 template <int N>
 hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool detect, long long part_dstride,
-                                 const float2* twy, const GrpArgs& ga, hipStream_t st);
+                                 const float2* twy, const GrpArgs& ga, bool panel, hipStream_t st);
 
 // frequency index of position i on an n-point grid in FFT order
 // (JMultiSlice.cpp:1699-1712)
@@ -360,12 +360,12 @@ int col_tiles(int ny, int nx) {
 hipError_t launch_row_pass(int nx, int ny, int nwave, float2* wave, long long wave_stride,
                            const float2* pgr, long long slc_off, long long npg, const int* var_tab,
                            int var_ld, int jslc, const float2* twx, const RowChanHost& rc, int mode,
-                           hipStream_t st) {
+                           bool panel, hipStream_t st) {
   switch (nx) {
 #define JMS_CASE(N)                                                                         \
   case N:                                                                                   \
     return launch_row_pass_n<N>(ny, nwave, wave, wave_stride, pgr, slc_off, npg, var_tab, \
-                                var_ld, jslc, twx, rc, mode, st);
+                                var_ld, jslc, twx, rc, mode, panel, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;
@@ -382,6 +382,23 @@ hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColAr
   }
 }
 
+bool panel_ok(int nx, int ny) {
+  bool rx = false, cy = false;
+  switch (nx) {
+#define JMS_CASE(N) case N: rx = Cfg<N>::PANEL; break;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return false;
+  }
+  switch (ny) {
+#define JMS_CASE(N) case N: cy = Cfg<N>::PANEL; break;
+    JMS_SIZES(JMS_CASE)
+#undef JMS_CASE
+    default: return false;
+  }
+  return rx && cy;
+}
+
 bool grp_supported(int nx, int ny) {
   if (!size_supported(nx)) return false;
   switch (ny) {
@@ -393,9 +410,9 @@ bool grp_supported(int nx, int ny) {
 }
 
 hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride, long long npg,
-                               const float2* twx, const GrpArgs& ga, hipStream_t st) {
+                               const float2* twx, const GrpArgs& ga, bool panel, hipStream_t st) {
   switch (nx) {
-#define JMS_CASE(N) case N: return launch_row_pass_grp_n<N>(ny, nwave, wave_stride, npg, twx, ga, st);
+#define JMS_CASE(N) case N: return launch_row_pass_grp_n<N>(ny, nwave, wave_stride, npg, twx, ga, panel, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;
@@ -404,10 +421,10 @@ hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride,
 
 hipError_t launch_col_pass_grp(int nx, int ny, int nwave, bool init, bool exit_, bool detect,
                                long long part_dstride, const float2* twy, const GrpArgs& ga,
-                               hipStream_t st) {
+                               bool panel, hipStream_t st) {
   switch (ny) {
 #define JMS_CASE(N) \
-  case N: return launch_col_pass_grp_n<N>(nx, nwave, init, exit_, detect, part_dstride, twy, ga, st);
+  case N: return launch_col_pass_grp_n<N>(nx, nwave, init, exit_, detect, part_dstride, twy, ga, panel, st);
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return hipErrorInvalidValue;

@@ -89,6 +89,16 @@ __device__ __forceinline__ void block_xy(int& bx, int& by) {
   }
 }
 
+// Panel layout of a wave (the staged batches where the line length allows
+// it): panels of kPanel columns, each NY rows x kPanel contiguous, so that a
+// column tile is one contiguous span and a row is NX / kPanel segments of
+// 128 bytes.  Element (y, x) sits at panel_off<NY>(x) + kPanel * y.
+constexpr int kPanel = 16;
+template <int NY>
+__device__ __forceinline__ int panel_off(int x) {
+  return (x & ~(kPanel - 1)) * NY + (x & (kPanel - 1));
+}
+
 // real-space readout channels of one detection plane (per wave: weight
 // cwgt[w], channel stride chan_wstride items, plane offset chan_off items)
 struct RowChan {
@@ -115,12 +125,13 @@ struct RowChan {
 //           step jslc come from its job's tables.
 // grid: (ceil(ny/LPB), nwave); block: T*LPB
 // ---------------------------------------------------------------------------
-template <int N, int MODE = 0, class C_ = Cfg<N>, bool GRP = false>
+template <int N, int MODE = 0, class C_ = Cfg<N>, bool GRP = false, bool PANEL = false>
 __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu(C_::WPE_ROW))) void k_row_pass(
     float2* __restrict__ wave, int ny, long long wave_stride, const float2* __restrict__ pgr,
     long long slc_off, long long npg, const int* __restrict__ var_tab, int var_ld, int jslc,
     const float2* __restrict__ twg, RowChan rc, GrpArgs ga) {
   static_assert(!GRP || MODE == 0, "grouped row passes carry no real-space readout");
+  static_assert(!PANEL || (MODE == 0 && N % kPanel == 0), "panel waves: whole panels, staged batches only");
   using P = typename C_::P;
   constexpr int T = C_::T, LPB = C_::LPB;
   using L = LineLayout<N, C_::PADSH_ROW, C_::SW>;
@@ -149,7 +160,9 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
   }
   const int row = bx * LPB + line;
   const bool valid = row < ny;
-  float2* wrow = wave + w * wave_stride + (long long)row * N;
+  // PANEL: element x of the row at (x / 16) * 16 ny + x % 16 from the row start
+  float2* wrow = wave + w * wave_stride + (long long)row * (PANEL ? kPanel : N);
+  auto wx = [&](int x) { return PANEL ? (x & ~(kPanel - 1)) * ny + (x & (kPanel - 1)) : x; };
   const int var = MODE == 2 ? 0 : var_tab[w * var_ld + jslc];
   const float2* grow = pgr + slc_off + (long long)var * npg + (long long)row * N;
   using M0 = IOMap<N, T, P, false>;
@@ -162,7 +175,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          v[i * M0::R0 + q] = ldg<C_::NT_ROW>(wrow + b + q * M0::NBF);
+          v[i * M0::R0 + q] = ldg<C_::NT_ROW>(wrow + wx(b + q * M0::NBF));
         });
       }
     });
@@ -214,7 +227,7 @@ __global__ __launch_bounds__(C_::T * C_::LPB) __attribute__((amdgpu_waves_per_eu
       if (M0::FULL || b < M0::NBF) {
         static_for<0, M0::R0>([&](auto Q) {
           constexpr int q = decltype(Q)::value;
-          stg<C_::NT_ROW>(wrow + b + q * M0::NBF, v[i * M0::R0 + q]);
+          stg<C_::NT_ROW>(wrow + wx(b + q * M0::NBF), v[i * M0::R0 + q]);
         });
       }
     });
@@ -708,7 +721,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, long long w, float2* 
 // cppl_t) so that the lanes of a column access them contiguously.
 // grid: (ceil(nx/CW), nwave); block: TW*CW
 // ---------------------------------------------------------------------------
-template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool GRP = false>
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool GRP = false, bool PANEL = false>
 __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu(C_::WPE_COL))) void k_col_pass_w(
     ColArgs a0) {
   constexpr int T = C_::TW, C = C_::CW, BS = T * C;
@@ -752,7 +765,11 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   const int nx = a.nx;
   const int x0 = blockIdx.x * C;
   const int ncol = (nx - x0 < C) ? (nx - x0) : C;
-  float2* wv_base = a.wave + w * a.wave_stride + x0;
+  // PANEL: the waves are stored in panels of 16 columns (element (y, x) at
+  // (x / 16) * 16 N + 16 y + x % 16), so the rows of a tile are contiguous
+  static_assert(!PANEL || C <= kPanel, "a panel holds whole tiles");
+  const int wrs = PANEL ? kPanel : nx;  // row stride of the wave
+  float2* wv_base = a.wave + w * a.wave_stride + (PANEL ? panel_off<N>(x0) : x0);
   // Fused tile I/O: when a column's first-stage butterflies are exactly the
   // T threads per column, the staging sweep already leaves in thread tid the
   // inputs of butterfly tid / C of column tid % C (rows b + k*N/R0), so the
@@ -767,7 +784,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
   const int cs = tid % C, bs = tid / C;  // sweep mapping: column, first-stage butterfly
   {
     float2 st[TL::KST];
-    TL::load(st, INIT ? (a.src + w * a.src_stride + x0) : wv_base, nx, ncol, tid);
+    TL::load(st, INIT ? (a.src + w * a.src_stride + x0) : wv_base, INIT ? nx : wrs, ncol, tid);
     if constexpr (FIN) {
       // twiddles from the global table: s_tw is complete only after the barrier
       LineFFT<N, T, -1, P, false, L, NREG, C_::TWR, 0>::run(st, s_buf, cs, a.twg, bs, SyncWave{});
@@ -786,7 +803,7 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
     } else {
       TL::from_lds(st, s_buf, tid);
     }
-    TL::store(st, wv_base, nx, ncol, tid);
+    TL::store(st, wv_base, wrs, ncol, tid);
   }
 }
 
@@ -852,7 +869,7 @@ __device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int 
   });
 }
 
-template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>>
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool PANEL = false>
 __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   constexpr int M = N / 64;
   static_assert(M == 16 || M == 32, "long-column kernel: N = 1024 or 2048");
@@ -873,7 +890,10 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   const int nx = a.nx;
   const int x0 = bx * CT;
   const int ncol = (nx - x0 < CT) ? (nx - x0) : CT;
-  float2* wbase = a.wave + w * a.wave_stride + x0;
+  // PANEL: the tile is one contiguous panel (element (y, x) at x0 N + 16 y + x - x0)
+  static_assert(!PANEL || CT == kPanel, "a panel is one tile");
+  const int wrs = PANEL ? kPanel : nx;  // row stride of the wave
+  float2* wbase = a.wave + w * a.wave_stride + (PANEL ? (long long)x0 * N : x0);
   // A layout (columns past the grid edge work on column 0 of the tile and
   // are not stored)
   const int ca = tid % CT, n1 = tid / CT;
@@ -902,12 +922,12 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   const int ob = kyb * nx + xb;
   const long long rstep = (long long)M * nx;  // elements from slot c to c + 1
   // A layout: row n1 + 64*n2 = (uniform row 64*n2) + per-thread offset
-  const long long astep = 64LL * nx;
+  const long long astep = 64LL * wrs;
   float2 v[M];  // A: v[n2] (rows n1 + 64*n2); B: v[16r + c]
   if constexpr (!INIT) {
     static_for<0, M>([&](auto I) {
       constexpr int n2 = decltype(I)::value;
-      v[n2] = ldg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * nx + xa));
+      v[n2] = ldg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * wrs + xa));
     });
     DFT<M, -1>::run(v);
   }
@@ -1013,10 +1033,12 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
     }
     if constexpr (EXIT) {
       if (validb) {
-        float2* wv0 = a.wave + w * a.wave_stride + 16 * r * nx;
+        float2* wv0 = PANEL ? wbase + 16 * r * kPanel : a.wave + w * a.wave_stride + 16 * r * nx;
+        const long long rs_w = PANEL ? (long long)M * kPanel : rstep;
+        const int ob_w = PANEL ? kyb * kPanel + cb : ob;
         static_for<0, 16>([&](auto K) {
           constexpr int c = decltype(K)::value;
-          stg<C_::NT_COL>(at32(wv0 + c * rstep, ob), u[c]);
+          stg<C_::NT_COL>(at32(wv0 + c * rs_w, ob_w), u[c]);
         });
       }
     }
@@ -1057,7 +1079,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
     if (ca < ncol) {
       static_for<0, M>([&](auto I) {
         constexpr int n2 = decltype(I)::value;
-        stg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * nx + ca), v[n2]);
+        stg<C_::NT_COL>(at32(wbase + n2 * astep, n1 * wrs + ca), v[n2]);
       });
     }
   }
@@ -1162,32 +1184,73 @@ template <int N>
 hipError_t launch_row_pass_n(int ny, int nwave, float2* wave, long long wave_stride,
                              const float2* pgr, long long slc_off, long long npg,
                              const int* var_tab, int var_ld, int jslc, const float2* twx,
-                             const RowChanHost& h, int mode, hipStream_t st) {
+                             const RowChanHost& h, int mode, bool panel, hipStream_t st) {
   dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nwave), b(Cfg<N>::T * Cfg<N>::LPB);
   RowChan rc;
   rc.img = h.img; rc.img_avg = h.img_avg;
   rc.wfr = (float2*)h.wfr; rc.wfr_avg = (float2*)h.wfr_avg;
   rc.cwgt = h.cwgt; rc.chan_wstride = h.chan_wstride; rc.chan_off = h.chan_off;
+  if (panel && (mode != 0 || !Cfg<N>::PANEL)) return hipErrorInvalidValue;
   if (mode == 1)
     hipLaunchKernelGGL((k_row_pass<N, 1>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
                        var_tab, var_ld, jslc, twx, rc, GrpArgs{});
   else if (mode == 2)
     hipLaunchKernelGGL((k_row_pass<N, 2>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
                        var_tab, var_ld, jslc, twx, rc, GrpArgs{});
-  else
+  else if (!panel)
     hipLaunchKernelGGL((k_row_pass<N, 0>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off, npg,
                        var_tab, var_ld, jslc, twx, rc, GrpArgs{});
+  else if constexpr (Cfg<N>::PANEL)
+    hipLaunchKernelGGL((k_row_pass<N, 0, Cfg<N>, false, true>), g, b, 0, st, wave, ny, wave_stride, pgr, slc_off,
+                       npg, var_tab, var_ld, jslc, twx, rc, GrpArgs{});
   return hipGetLastError();
 }
 
 // grouped row pass (waves of several jobs, one slice step)
 template <int N>
 hipError_t launch_row_pass_grp_n(int ny, int nwave, long long wave_stride, long long npg, const float2* twx,
-                                 const GrpArgs& ga, hipStream_t st) {
+                                 const GrpArgs& ga, bool panel, hipStream_t st) {
   dim3 g((ny + Cfg<N>::LPB - 1) / Cfg<N>::LPB, nwave), b(Cfg<N>::T * Cfg<N>::LPB);
-  hipLaunchKernelGGL((k_row_pass<N, 0, Cfg<N>, true>), g, b, 0, st, nullptr, ny, wave_stride, nullptr, 0LL, npg,
-                     nullptr, 0, ga.step, twx, RowChan{}, ga);
+  if (panel && !Cfg<N>::PANEL) return hipErrorInvalidValue;
+  if (!panel)
+    hipLaunchKernelGGL((k_row_pass<N, 0, Cfg<N>, true>), g, b, 0, st, nullptr, ny, wave_stride, nullptr, 0LL, npg,
+                       nullptr, 0, ga.step, twx, RowChan{}, ga);
+  else if constexpr (Cfg<N>::PANEL)
+    hipLaunchKernelGGL((k_row_pass<N, 0, Cfg<N>, true, true>), g, b, 0, st, nullptr, ny, wave_stride, nullptr, 0LL,
+                       npg, nullptr, 0, ga.step, twx, RowChan{}, ga);
   return hipGetLastError();
+}
+
+// the column kernels' instantiations by pass kind (INIT, EXIT, readout)
+template <int N, class CF, bool GRP, bool PANEL>
+void launch_col_w(dim3 g, dim3 b, bool init, bool exit_, bool detect, const ColArgs& a, hipStream_t st) {
+  // detection planes get their own instantiation: the readout's registers
+  // would otherwise halve the occupancy of every other slice
+  if (detect) {
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, true, CF, GRP, PANEL>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, true, CF, GRP, PANEL>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, true, CF, GRP, PANEL>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_w<N, false, false, true, CF, GRP, PANEL>), g, b, 0, st, a);
+  } else {
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, false, CF, GRP, PANEL>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, false, CF, GRP, PANEL>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, false, CF, GRP, PANEL>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, GRP, PANEL>), g, b, 0, st, a);
+  }
+}
+template <int N, class CF, bool PANEL>
+void launch_col_x(dim3 g, dim3 b, bool init, bool exit_, bool detect, const ColArgs& a, hipStream_t st) {
+  if (detect) {
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, true, CF, PANEL>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, true, CF, PANEL>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, true, CF, PANEL>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, true, CF, PANEL>), g, b, 0, st, a);
+  } else {
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, false, CF, PANEL>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, false, CF, PANEL>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, false, CF, PANEL>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL>), g, b, 0, st, a);
+  }
 }
 
 template <int N>
@@ -1208,35 +1271,17 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   a.detw_t = h.detw_t;
   a.cppl_t = h.cppl_t;
   using CF = Cfg<N>;
+  if (h.panel && !CF::PANEL) return hipErrorInvalidValue;
   if constexpr (CF::XL) {
     dim3 g((h.nx + 15) / 16, nwave), b(1024);
-    if (h.detect) {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, true>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, true>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_x<N, false, false, true>), g, b, 0, st, a);
-    } else {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, false>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, false>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, false>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_x<N, false, false, false>), g, b, 0, st, a);
-    }
+    if (!h.panel) launch_col_x<N, CF, false>(g, b, init, exit_, h.detect, a, st);
+    else if constexpr (CF::PANEL) launch_col_x<N, CF, true>(g, b, init, exit_, h.detect, a, st);
   } else if constexpr (CF::WL) {
     dim3 g((h.nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
-    // detection planes get their own instantiation: the readout's registers
-    // would otherwise halve the occupancy of every other slice
-    if (h.detect) {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, true>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, true>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, true>), g, b, 0, st, a);
-    } else {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, false>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, false>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, false>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false>), g, b, 0, st, a);
-    }
+    if (!h.panel) launch_col_w<N, CF, false, false>(g, b, init, exit_, h.detect, a, st);
+    else if constexpr (CF::PANEL) launch_col_w<N, CF, false, true>(g, b, init, exit_, h.detect, a, st);
   } else {
+    if (h.panel) return hipErrorInvalidValue;
     dim3 g((h.nx + CF::C - 1) / CF::C, nwave), b(CF::TC * CF::C);
     if (init && exit_) hipLaunchKernelGGL((k_col_pass<N, true, true>), g, b, 0, st, a);
     else if (init) hipLaunchKernelGGL((k_col_pass<N, true, false>), g, b, 0, st, a);
@@ -1249,9 +1294,10 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
 // grouped column pass (waves of several jobs; the wave-local kernel only)
 template <int N>
 hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool detect, long long part_dstride,
-                                 const float2* twy, const GrpArgs& ga, hipStream_t st) {
+                                 const float2* twy, const GrpArgs& ga, bool panel, hipStream_t st) {
   using CF = Cfg<N>;
   if constexpr (CF::WL && !CF::XL) {
+    if (panel && !CF::PANEL) return hipErrorInvalidValue;
     ColArgs a{};
     a.nx = nx;
     a.wave_stride = (long long)N * nx;
@@ -1259,17 +1305,8 @@ hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool 
     a.twg = twy;
     a.grp = ga;
     dim3 g((nx + CF::CW - 1) / CF::CW, nwave), b(CF::TW * CF::CW);
-    if (detect) {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, true, CF, true>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, true, CF, true>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, true, CF, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, true, CF, true>), g, b, 0, st, a);
-    } else {
-      if (init && exit_) hipLaunchKernelGGL((k_col_pass_w<N, true, true, false, CF, true>), g, b, 0, st, a);
-      else if (init) hipLaunchKernelGGL((k_col_pass_w<N, true, false, false, CF, true>), g, b, 0, st, a);
-      else if (exit_) hipLaunchKernelGGL((k_col_pass_w<N, false, true, false, CF, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, true>), g, b, 0, st, a);
-    }
+    if (!panel) launch_col_w<N, CF, true, false>(g, b, init, exit_, detect, a, st);
+    else if constexpr (CF::PANEL) launch_col_w<N, CF, true, true>(g, b, init, exit_, detect, a, st);
     return hipGetLastError();
   } else {
     return hipErrorInvalidValue;

@@ -6,7 +6,8 @@
 // wave-local column block), WPE (waves per SIMD asked of the register
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
 // default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
-// none), PADR (the same for the row pass).  Included by jms_cfg.hpp after the plans.
+// none), PADR (the same for the row pass), PANEL (waves of staged batches in
+// 16-column panels).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
@@ -32,6 +33,7 @@ struct Tuned {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 
 template <> struct TunedColPlan<360> { using type = Radices<9, 5, 8>; };
@@ -49,6 +51,7 @@ template <> struct Tuned<16> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<48> {
   static constexpr int LPB = 64;
@@ -58,6 +61,7 @@ template <> struct Tuned<48> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<64> {
   static constexpr int LPB = 64;
@@ -67,6 +71,7 @@ template <> struct Tuned<64> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<96> {
   static constexpr int LPB = 42;
@@ -76,6 +81,7 @@ template <> struct Tuned<96> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<128> {
   static constexpr int LPB = 16;
@@ -85,6 +91,7 @@ template <> struct Tuned<128> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<160> {
   static constexpr int LPB = 16;
@@ -94,6 +101,7 @@ template <> struct Tuned<160> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<192> {
   static constexpr int LPB = 8;
@@ -103,6 +111,7 @@ template <> struct Tuned<192> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<256> {
   static constexpr int LPB = 8;
@@ -112,6 +121,7 @@ template <> struct Tuned<256> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<320> {
   static constexpr int LPB = 8;
@@ -121,6 +131,7 @@ template <> struct Tuned<320> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
@@ -130,6 +141,7 @@ template <> struct Tuned<360> {
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
   static constexpr int PADR = 24;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<384> {
   static constexpr int LPB = 8;
@@ -139,6 +151,7 @@ template <> struct Tuned<384> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = true;
 };
 template <> struct Tuned<480> {
   static constexpr int LPB = 6;
@@ -148,6 +161,7 @@ template <> struct Tuned<480> {
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
   static constexpr int PADR = 24;
+  static constexpr bool PANEL = true;
 };
 template <> struct Tuned<512> {
   static constexpr int LPB = 4;
@@ -157,6 +171,7 @@ template <> struct Tuned<512> {
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = true;
 };
 template <> struct Tuned<625> {
   static constexpr int LPB = 4;
@@ -166,6 +181,7 @@ template <> struct Tuned<625> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -175,6 +191,7 @@ template <> struct Tuned<640> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 24;
+  static constexpr bool PANEL = true;
 };
 template <> struct Tuned<720> {
   static constexpr int LPB = 2;
@@ -184,6 +201,7 @@ template <> struct Tuned<720> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 24;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
@@ -193,6 +211,7 @@ template <> struct Tuned<768> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
@@ -202,6 +221,7 @@ template <> struct Tuned<1024> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
@@ -211,6 +231,7 @@ template <> struct Tuned<2048> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
+  static constexpr bool PANEL = false;
 };
 
 }  // namespace jms

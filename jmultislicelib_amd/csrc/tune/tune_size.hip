@@ -54,7 +54,7 @@ struct Bufs {
   int nw;
 };
 
-template <class CF>
+template <class CF, bool PANEL = false>
 float time_row(const Bufs& b, int reps) {
   dim3 g((N + CF::LPB - 1) / CF::LPB, b.nw), bl(CF::T * CF::LPB);
   RowChan rc{};
@@ -63,8 +63,8 @@ float time_row(const Bufs& b, int reps) {
   (void)hipEventCreate(&e1);
   (void)hipEventRecord(e0);
   for (int i = 0; i < reps; ++i)
-    hipLaunchKernelGGL((k_row_pass<N, 0, CF>), g, bl, 0, 0, b.w, N, (long long)N * N, b.pg, 0LL, (long long)N * N,
-                       b.var, 1, 0, b.tw, rc, GrpArgs{});
+    hipLaunchKernelGGL((k_row_pass<N, 0, CF, false, PANEL>), g, bl, 0, 0, b.w, N, (long long)N * N, b.pg, 0LL,
+                       (long long)N * N, b.var, 1, 0, b.tw, rc, GrpArgs{});
   (void)hipEventRecord(e1);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
@@ -73,7 +73,7 @@ float time_row(const Bufs& b, int reps) {
 }
 
 // column pass of a middle slice with the kernel the launcher would pick for CF
-template <class CF>
+template <class CF, bool PANEL = false>
 float time_col(const Bufs& b, int reps) {
   ColArgs a{};
   a.wave = b.w; a.nx = N; a.wave_stride = (long long)N * N; a.pro = b.pro; a.pro_t = b.pro; a.twg = b.tw;
@@ -83,9 +83,10 @@ float time_col(const Bufs& b, int reps) {
   (void)hipEventRecord(e0);
   for (int i = 0; i < reps; ++i) {
     if constexpr (CF::XL) {
-      hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF>), dim3((N + 15) / 16, b.nw), dim3(1024), 0, 0, a);
+      hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL>), dim3((N + 15) / 16, b.nw), dim3(1024), 0,
+                         0, a);
     } else if constexpr (CF::WL) {
-      hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF>), dim3((N + CF::CW - 1) / CF::CW, b.nw),
+      hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, false, PANEL>), dim3((N + CF::CW - 1) / CF::CW, b.nw),
                          dim3(CF::TW * CF::CW), 0, 0, a);
     } else {
       hipLaunchKernelGGL((k_col_pass<N, false, false, CF>), dim3((N + CF::C - 1) / CF::C, b.nw),
@@ -157,6 +158,17 @@ int main() {
   constexpr int L0 = (256 / B::T) > 0 ? (256 / B::T) : 1;  // default rows per block
   constexpr int L2 = (B::T * 2 * L0 <= 1024) ? 2 * L0 : L0;
   constexpr int LH = L0 > 1 ? L0 / 2 : 1;
+  if constexpr (N % 16 == 0) {  // waves in 16-column panels: the tuned row and column kernels
+    float br = 1e30f, bp = 1e30f, cr = 1e30f, cp = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      br = std::fmin(br, time_row<B, false>(b, 10));
+      bp = std::fmin(bp, time_row<B, true>(b, 10));
+      cr = std::fmin(cr, time_col<B, false>(b, 10));
+      cp = std::fmin(cp, time_col<B, true>(b, 10));
+    }
+    std::printf("{\"n\": %d, \"pass\": \"layout\", \"row_ms\": %.5f, \"row_panel_ms\": %.5f, \"col_ms\": %.5f, "
+                "\"col_panel_ms\": %.5f}\n", N, br, bp, cr, cp);
+  }
   row_variant<RowP<3>>(b, "tuned", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowP<24>>(b, "tuned_flat", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowP<2>>(b, "tuned_pad2", B::LPB, B::GTW_ROW, bytes);
