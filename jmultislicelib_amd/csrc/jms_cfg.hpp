@@ -27,7 +27,10 @@ struct CfgP {
     for (int s = 0; s < Q::S; ++s) m = (N / Q::R[s] > m) ? N / Q::R[s] : m;
     return m;
   }
-  static constexpr int T = max_bfly_of<P>();             // row pass: threads per line
+  // row pass: threads per line, one per butterfly of the widest stage, or
+  // (TR, jms_tuned.h) fewer with several butterflies each, e.g. 64 so that a
+  // row is one wavefront and its exchanges need wave barriers only
+  static constexpr int T = (Tuned<N>::TR > 0 && Tuned<N>::TR < max_bfly_of<P>()) ? Tuned<N>::TR : max_bfly_of<P>();
   static constexpr int TCOL = max_bfly_of<PC>();         // butterflies per column stage (at most)
   // rows per block: 256 threads' worth, or the tuned value (jms_tuned.h)
   // when it fits a block with this plan's threads per row

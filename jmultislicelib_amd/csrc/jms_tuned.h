@@ -7,7 +7,7 @@
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
 // default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
 // none), PADR (the same for the row pass), PANEL (waves of staged batches in
-// 16-column panels).  Included by jms_cfg.hpp after the plans.
+// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
@@ -34,6 +34,7 @@ struct Tuned {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 
 template <> struct TunedColPlan<360> { using type = Radices<9, 5, 8>; };
@@ -52,6 +53,7 @@ template <> struct Tuned<16> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<48> {
   static constexpr int LPB = 64;
@@ -62,6 +64,7 @@ template <> struct Tuned<48> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<64> {
   static constexpr int LPB = 64;
@@ -72,6 +75,7 @@ template <> struct Tuned<64> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<96> {
   static constexpr int LPB = 42;
@@ -82,6 +86,7 @@ template <> struct Tuned<96> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<128> {
   static constexpr int LPB = 16;
@@ -92,6 +97,7 @@ template <> struct Tuned<128> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<160> {
   static constexpr int LPB = 16;
@@ -102,6 +108,7 @@ template <> struct Tuned<160> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<192> {
   static constexpr int LPB = 8;
@@ -112,6 +119,7 @@ template <> struct Tuned<192> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<256> {
   static constexpr int LPB = 8;
@@ -122,6 +130,7 @@ template <> struct Tuned<256> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<320> {
   static constexpr int LPB = 8;
@@ -132,6 +141,7 @@ template <> struct Tuned<320> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
@@ -142,6 +152,7 @@ template <> struct Tuned<360> {
   static constexpr int PADC = 24;
   static constexpr int PADR = 24;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<384> {
   static constexpr int LPB = 8;
@@ -152,9 +163,10 @@ template <> struct Tuned<384> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = true;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<480> {
-  static constexpr int LPB = 6;
+  static constexpr int LPB = 4;
   static constexpr bool GTW = false;
   static constexpr int CW = 8;
   static constexpr int WPE = 1;
@@ -162,6 +174,7 @@ template <> struct Tuned<480> {
   static constexpr int PADC = 24;
   static constexpr int PADR = 24;
   static constexpr bool PANEL = true;
+  static constexpr int TR = 64;
 };
 template <> struct Tuned<512> {
   static constexpr int LPB = 4;
@@ -172,6 +185,7 @@ template <> struct Tuned<512> {
   static constexpr int PADC = 24;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = true;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<625> {
   static constexpr int LPB = 4;
@@ -182,6 +196,7 @@ template <> struct Tuned<625> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -192,6 +207,7 @@ template <> struct Tuned<640> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 24;
   static constexpr bool PANEL = true;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<720> {
   static constexpr int LPB = 2;
@@ -202,6 +218,7 @@ template <> struct Tuned<720> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 24;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
@@ -212,6 +229,7 @@ template <> struct Tuned<768> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
@@ -222,6 +240,7 @@ template <> struct Tuned<1024> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
@@ -232,6 +251,7 @@ template <> struct Tuned<2048> {
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
+  static constexpr int TR = 0;
 };
 
 }  // namespace jms

@@ -125,6 +125,9 @@ def choose(results):
         tuned = {r["variant"]: r for r in rs if r["pass"] == "row" and r["variant"].startswith("tuned")}
         if "tuned" in tuned and "tuned_flat" in tuned and tuned["tuned_flat"]["ms"] < 0.98 * tuned["tuned"]["ms"]:
             k["padr"] = 24
+        lay = next((r for r in rs if r["pass"] == "layout"), None)
+        if lay and lay["row_panel_ms"] + lay["col_panel_ms"] < 0.98 * (lay["row_ms"] + lay["col_ms"]):
+            k["panel"] = True
         rows = [r for r in rs if r["pass"] == "row" and not r["variant"].startswith("tuned")]
         cols = [r for r in rs if r["pass"] == "col"]
         k = {}
@@ -133,6 +136,8 @@ def choose(results):
             b = min(rows, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
                 k["lpb"], k["gtw"] = b["lpb"], b["gtw"]
+                if b["variant"] == "tr64":
+                    k["tr"] = 64
         if cols:
             cols = [r for r in cols if r.get("twr", 1) == 1]  # per-leg twiddle reads: timed, not selectable
             d = cols[0]  # the first variant timed: the current configuration
@@ -185,6 +190,8 @@ def emit(results, path):
              "  static constexpr int TW = 0;",
              "  static constexpr int PADC = 0;",
              "  static constexpr int PADR = 0;",
+             "  static constexpr bool PANEL = false;",
+             "  static constexpr int TR = 0;",
              "};", ""]
     for n, k in ch.items():
         if "col_plan" in k:
@@ -203,6 +210,8 @@ def emit(results, path):
         lines.append(f"  static constexpr int TW = {k.get('tw', 0)};")
         lines.append(f"  static constexpr int PADC = {0 if k.get('pad', 3) == 3 else k['pad']};")
         lines.append(f"  static constexpr int PADR = {k.get('padr', 0)};")
+        lines.append(f"  static constexpr bool PANEL = {'true' if k.get('panel') else 'false'};")
+        lines.append(f"  static constexpr int TR = {k.get('tr', 0)};")
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))

@@ -21,6 +21,12 @@ struct RowV : B {
   static constexpr bool GTW_ROW = G;
   static constexpr bool TWR_ROW = G;
 };
+// rows on TRV threads (several butterflies per thread where a stage has more)
+template <int TRV, int LPBV>
+struct RowT : B {
+  static constexpr int T = TRV;
+  static constexpr int LPB = LPBV;
+};
 // the tuned row shape with a given LDS padding shift (3: one slot per 8
 // elements; >= 20: none)
 template <int PADV>
@@ -168,6 +174,25 @@ int main() {
     }
     std::printf("{\"n\": %d, \"pass\": \"layout\", \"row_ms\": %.5f, \"row_panel_ms\": %.5f, \"col_ms\": %.5f, "
                 "\"col_panel_ms\": %.5f}\n", N, br, bp, cr, cp);
+    // panel rows with twice and half the rows per block (longer / shorter contiguous runs per panel)
+    constexpr int LX = 2 * B::LPB, LH2 = B::LPB > 1 ? B::LPB / 2 : 1;
+    float b2 = 1e30f, bh = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      if constexpr (B::T * LX <= 1024) b2 = std::fmin(b2, time_row<RowV<LX, B::GTW_ROW>, true>(b, 10));
+      bh = std::fmin(bh, time_row<RowV<LH2, B::GTW_ROW>, true>(b, 10));
+    }
+    std::printf("{\"n\": %d, \"pass\": \"panel_rows\", \"lpb\": %d, \"ms\": %.5f, \"lpb_x2\": %d, \"ms_x2\": %.5f, "
+                "\"lpb_half\": %d, \"ms_half\": %.5f}\n", N, B::LPB, bp, LX, b2, LH2, bh);
+  }
+  if constexpr (B::T > 64) {  // one wavefront per row
+    row_variant<RowT<64, 4>>(b, "tr64", 4, B::GTW_ROW, bytes);
+    row_variant<RowT<64, 2>>(b, "tr64", 2, B::GTW_ROW, bytes);
+    row_variant<RowT<64, 8>>(b, "tr64", 8, B::GTW_ROW, bytes);
+    if constexpr (N % 16 == 0) {
+      float bp = 1e30f;
+      for (int r = 0; r < 3; ++r) bp = std::fmin(bp, time_row<RowT<64, 4>, true>(b, 10));
+      std::printf("{\"n\": %d, \"pass\": \"tr64_panel\", \"lpb\": 4, \"ms\": %.5f}\n", N, bp);
+    }
   }
   row_variant<RowP<3>>(b, "tuned", B::LPB, B::GTW_ROW, bytes);
   row_variant<RowP<24>>(b, "tuned_flat", B::LPB, B::GTW_ROW, bytes);
