@@ -55,10 +55,26 @@ struct CfgP {
   static constexpr bool WL = (N <= 768) && ((TW * CW) % 64 == 0);
   // long columns (N = 1024, 2048): register-resident column kernel with 16
   // columns per block (128-byte row segments; a 2048^2 column-tile copy runs
-  // at 0.93 / 2.6 / 5.1 TB/s with 4 / 8 / 16 columns per block on MI355X)
+  // at 0.93 / 2.6 / 5.1 TB/s with 4 / 8 / 16 columns per block on MI355X in
+  // the hardware block order), or 8 (CW = 8, jms_tuned.h) in blocks of 512
+  // threads, two per CU, ordered so that the neighbouring tiles sharing a
+  // 128-byte line run on one XCD at the same time
   static constexpr bool XL = (N == 1024 || N == 2048);
+  static constexpr int XCT = (XL && CW == 8) ? 8 : 16;
+  // block order of the long-column kernel: XG_X > 0 takes block_xy_groups
+  // with groups of XG_X waves rotated by XROT_X tiles, 0 the hardware order.
+  // Per 1 GiB of waves on MI355X (tuner): N = 1024 16 columns 0.706 ms, 8
+  // columns in the hardware order 0.793 ms, in groups of 1 / 2 / 4 waves
+  // 0.707 / 0.597 / 0.618 ms; N = 2048 16 columns 0.706 ms, 8 columns
+  // 0.70-1.05 ms in any of these orders
+  static constexpr int XG_X = XCT == 8 ? 2 : 0;
+  static constexpr int XROT_X = XCT == 8 ? 16 : 0;
+  // A-phase twiddles of the long-column kernel read whole from a full table
+  // (16-column blocks) instead of two half-table reads and a product: 6%
+  // fewer VALU instructions, yet 0.706 -> 0.723 ms at N = 2048 (off)
+  static constexpr bool XTWF = false;
   // columns per block of the column kernel in use (partial-sum tiles)
-  static constexpr int CC = XL ? 16 : (WL ? CW : C);
+  static constexpr int CC = XL ? XCT : (WL ? CW : C);
   static constexpr int PADSH = 3;
   // LDS padding of the wave-local column kernel's tile (a shift >= 20: no
   // padding, every exchange address a constant offset from one base)

@@ -74,19 +74,37 @@ __device__ __forceinline__ int fidx(int i, int n) {
 // the grating row or propagator tile of bx enters that XCD's L2 once and
 // serves every wave.  A bijection of the grid for any size; identity
 // without XCD.
+__device__ __forceinline__ unsigned xcd_block_id() {
+  const unsigned nb = gridDim.x * gridDim.y;
+  const unsigned b = blockIdx.x + gridDim.x * blockIdx.y;
+  const unsigned x8 = b & 7u, q = nb >> 3, r = nb & 7u;
+  return x8 * q + (x8 < r ? x8 : r) + (b >> 3);
+}
 template <bool XCD>
 __device__ __forceinline__ void block_xy(int& bx, int& by) {
   if constexpr (XCD) {
-    const unsigned nbx = gridDim.x, nby = gridDim.y, nb = nbx * nby;
-    const unsigned b = blockIdx.x + nbx * blockIdx.y;
-    const unsigned x8 = b & 7u, q = nb >> 3, r = nb & 7u;
-    const unsigned id = x8 * q + (x8 < r ? x8 : r) + (b >> 3);
+    const unsigned id = xcd_block_id(), nby = gridDim.y;
     bx = (int)(id / nby);
     by = (int)(id % nby);
   } else {
     bx = (int)blockIdx.x;
     by = (int)blockIdx.y;
   }
+}
+// The same consecutive ids per XCD over groups of G block rows: within a
+// group the id runs over by fastest, then bx, so the blocks in flight on one
+// XCD are G block rows (waves) of a few neighbouring block columns (tiles):
+// with G = 1 tiles that share 128-byte lines meet in one L2, with G > 1 a
+// tile's propagator columns serve G waves from it.  Group gi starts at block
+// column (gi * ROT) mod gridDim.x (a bijection of each group), so that the
+// XCDs, each on its own groups, work at different column offsets.
+template <int G, int ROT>
+__device__ __forceinline__ void block_xy_groups(int& bx, int& by) {
+  const unsigned id = xcd_block_id(), nbx = gridDim.x, nby = gridDim.y;
+  const unsigned gi = id / (G * nbx), rem = id % (G * nbx);
+  const unsigned g0 = gi * G, ge = (nby - g0 < (unsigned)G) ? nby - g0 : (unsigned)G;
+  by = (int)(g0 + rem % ge);
+  bx = (int)((rem / ge + gi * ROT) % nbx);
 }
 
 // Panel layout of a wave (the staged batches where the line length allows
@@ -809,23 +827,29 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
 
 // ---------------------------------------------------------------------------
 // column pass, register-resident form for long columns, N = 64*M with M = 16
-// or 32 (N = 1024, 2048).  A block of 1024 threads owns 16 adjacent columns,
-// so every row segment it reads or writes is 128 bytes; the tile (256 KiB at
-// N = 2048, more than the LDS holds) stays in registers, M elements per
-// thread, and the column transform is split as N = 64 * M (four-step):
-//   A layout (global loads and stores): thread tid holds column tid % 16,
-//     rows n1 + 64*n2 (n1 = tid / 16, n2 = 0..M-1); an M-point DFT over n2 in
+// or 32 (N = 1024, 2048).  A block of 64*CT threads owns CT = 16 adjacent
+// columns (every row segment it reads or writes is 128 bytes, one block per
+// CU) or CT = 8 (64-byte segments; blocks of 512 threads, two per CU, so
+// that one block's transforms overlap the other's memory phase, in an
+// XCD-aware order that runs both halves of a 128-byte line in one L2); the
+// tile (256 KiB at N = 2048, CT = 16: more than the LDS holds) stays in
+// registers, M elements per thread, and the column transform is split as
+// N = 64 * M (four-step):
+//   A layout (global loads and stores): thread tid holds column tid % CT,
+//     rows n1 + 64*n2 (n1 = tid / CT, n2 = 0..M-1); an M-point DFT over n2 in
 //     registers, then the twiddle W_N^(n1*k2);
-//   exchange through LDS in rounds of 16 values of k2 (128 KiB per round);
-//   B layout: wavefront wv of round r holds k2 = 16r + wv; the lanes of a
-//     quad (j = lane % 4) hold column lane / 4, slot m <-> n1 = j + 4m; a
-//     16-point DFT over m in registers, the twiddle W_64^(j*c), and a 4-point
-//     DFT across the quad by two DPP butterflies.  Slot c of lane
-//     j = j0 + 2*j1 then holds frequency ky = k2 + M*(c + 16*(j1 + 2*j0)).
+//   exchange through LDS in rounds of 16 values of k2 (8 KiB per column);
+//   B layout: k2 = 16r + kk in round r, kk = wavefront wv (CT = 16) or
+//     2 wv + lane / 32 (CT = 8); the lanes of a quad (j = lane % 4) hold
+//     column (lane / 4) % CT, slot m <-> n1 = j + 4m; a 16-point DFT over m
+//     in registers, the twiddle W_64^(j*c), and a 4-point DFT across the
+//     quad by two DPP butterflies.  Slot c of lane j = j0 + 2*j1 then holds
+//     frequency ky = k2 + M*(c + 16*(j1 + 2*j0)).
 // The phase plate (INIT), propagator, readouts and the exit store act in the
-// B layout on row-major tables (a wavefront touches four 128-byte row
-// segments per access); the inverse transform runs the steps backwards.
-// grid: (ceil(nx/16), nwave); block: 1024
+// B layout on row-major tables (a wavefront touches four 128-byte or eight
+// 64-byte row segments per access); the inverse transform runs the steps
+// backwards.
+// grid: (ceil(nx/CT), nwave); block: 64*CT
 // ---------------------------------------------------------------------------
 // the value of x, hidden from the optimiser: table reads indexed with it are
 // repeated where they are needed instead of being kept live in registers
@@ -847,9 +871,12 @@ __device__ __forceinline__ const T* at32(const T* base, int idx) {
 }
 
 // v[k] *= W_N^(DIR * n1 * k), k = 1..M-1, N = 64*M, from two exact table
-// reads per factor: W^(n1*(8a + b)) = W^(8a*n1) * W^(b*n1) (n1*k < N)
+// reads per factor: W^(n1*(8a + b)) = W^(8a*n1) * W^(b*n1) (n1*k < N); tw
+// holds the first half of the table, W^(e + N/2) = -W^e
 template <int M, int DIR>
 __device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int n1) {
+  constexpr int H = 32 * M;
+  static_assert(7 * 63 < H, "W^(b*n1) in the half table");
   float2 wb[8];
   static_for<1, 8>([&](auto B) {
     constexpr int b = decltype(B)::value;
@@ -857,7 +884,10 @@ __device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int 
   });
   static_for<0, M / 8>([&](auto A) {
     constexpr int a8 = decltype(A)::value;
-    const float2 wa = tw[8 * a8 * n1];
+    const int e = 8 * a8 * n1;
+    const bool hi = e >= H;
+    const float2 th = tw[hi ? e - H : e];
+    const float2 wa = hi ? make_float2(-th.x, -th.y) : th;
     static_for<0, 8>([&](auto B) {
       constexpr int b = decltype(B)::value;
       constexpr int k = 8 * a8 + b;
@@ -869,23 +899,43 @@ __device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int 
   });
 }
 
+// the same from a per-thread row of the table W_N^(n1*k), k = 0..M-1 (one
+// read per factor at a constant offset)
+template <int M, int DIR>
+__device__ __forceinline__ void twiddle_a_row(float2 (&v)[M], const float2* row) {
+  static_for<1, M>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    v[k] = (DIR < 0) ? cmul(v[k], row[k]) : cmulconj(v[k], row[k]);
+  });
+}
+
 template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool PANEL = false>
-__global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
+__global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4))) void k_col_pass_x(ColArgs a) {
   constexpr int M = N / 64;
   static_assert(M == 16 || M == 32, "long-column kernel: N = 1024 or 2048");
   constexpr int R = M / 16;   // exchange rounds
-  constexpr int CT = 16;      // columns per block
+  constexpr int CT = C_::XCT;  // columns per block
+  static_assert(CT == 16 || CT == 8, "long-column kernel: 8 or 16 columns per block");
+  constexpr int NT = 64 * CT;  // threads
   __shared__ float2 s_x[16 * 64 * CT];  // one round: [k2 % 16][n1][column]
-  __shared__ float2 s_tw[N];
+  // TWF: W_N^(n1*k) at [n1][k] (rows padded to M + 1: the four values of
+  // n1 in a wavefront fall on distinct banks); else W_N^e, e < N/2
+  constexpr bool TWF = C_::XTWF && CT == 16;
+  __shared__ float2 s_tw[TWF ? 64 * (M + 1) : N / 2];
   // W_64^(j*c) at [c][j] (j = 0..3, c = 0..15): the four lanes of a quad
   // read four adjacent entries (in s_tw they would share one bank)
   __shared__ float2 s_w64[64];
   __shared__ float s_red[16 * kMaxDet];
   const int tid = threadIdx.x;
-  for (int e = tid; e < N; e += 1024) s_tw[e] = a.twg[e];
+  if constexpr (TWF) {
+    for (int e = tid; e < 64 * M; e += NT) s_tw[(e / M) * (M + 1) + e % M] = a.twg[(e / M) * (e % M)];
+  } else {
+    for (int e = tid; e < N / 2; e += NT) s_tw[e] = a.twg[e];
+  }
   if (tid < 64) s_w64[tid] = a.twg[((tid & 3) * (tid >> 2) * M) % N];
   int bx, by;
-  block_xy<C_::XCD_COL>(bx, by);
+  if constexpr (C_::XG_X > 0) block_xy_groups<C_::XG_X, C_::XROT_X>(bx, by);
+  else block_xy<C_::XCD_COL>(bx, by);
   const long long w = by;
   const int nx = a.nx;
   const int x0 = bx * CT;
@@ -900,22 +950,24 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   const int xa = (ca < ncol) ? ca : 0;
   // B layout
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  const int j = lane & 3, cb = lane >> 2;
+  const int kk = CT == 16 ? wv : 2 * wv + (lane >> 5);  // k2 % 16
+  const int j = lane & 3, cb = (lane >> 2) & (CT - 1);
   const int j0 = j & 1, j1 = j >> 1;
   const int dq = j1 + 2 * j0;
   const float sg1 = j0 ? -1.f : 1.f, sg2 = j1 ? -1.f : 1.f;
   const bool validb = cb < ncol;
   const int xb = x0 + (validb ? cb : 0);
-  // exchange buffer [k2 % 16][n1][column], columns XOR-swizzled: rows with
-  // bit 1 set hold column c at c ^ 8.  The B-layout accesses (rows j + 4m of
-  // a quad's four lanes, 8 columns per half-wavefront) then fall on 64
-  // distinct banks instead of two lanes per bank; the A-layout accesses
-  // stay a permutation inside one row
-  static_assert(CT == 16, "swizzle assumes 16 columns");
-  const int swa = ca ^ (((n1 >> 1) & 1) << 3);
-  const int swb = cb ^ (j1 << 3);
+  // exchange buffer [k2 % 16][n1][column].  CT = 16: columns XOR-swizzled,
+  // rows with bit 1 set hold column c at c ^ 8, so that the B-layout
+  // accesses (rows j + 4m of a quad's four lanes, 8 columns per
+  // half-wavefront) fall on 64 distinct banks instead of two lanes per bank;
+  // the A-layout accesses stay a permutation inside one row.  CT = 8: the
+  // four rows of a half-wavefront's quads are 64 bytes apart, distinct banks
+  // as they lie
+  const int swa = CT == 16 ? ca ^ (((n1 >> 1) & 1) << 3) : ca;
+  const int swb = CT == 16 ? cb ^ (j1 << 3) : cb;
   // slot (r, c) of the B layout <-> row ky = 16r + M*c (uniform part) + kyb
-  const int kyb = wv + 16 * M * dq;
+  const int kyb = kk + 16 * M * dq;
   auto kyof = [&](int r, int c) { return 16 * r + M * c + kyb; };
   // rows of the B layout are addressed as (row 16r + M*c of a table) +
   // (per-lane offset kyb*nx + x): the first part is wavefront-uniform
@@ -933,7 +985,10 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
   }
   __builtin_amdgcn_sched_barrier(0);
   __syncthreads();  // twiddle table complete
-  if constexpr (!INIT) twiddle_a<M, -1>(v, s_tw, opaque(n1));
+  if constexpr (!INIT) {
+    if constexpr (TWF) twiddle_a_row<M, -1>(v, s_tw + n1 * (M + 1));
+    else twiddle_a<M, -1>(v, s_tw, opaque(n1));
+  }
   float acc[kMaxDet];
 #pragma unroll
   for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
@@ -975,7 +1030,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       __syncthreads();
       static_for<0, 16>([&](auto K) {
         constexpr int m = decltype(K)::value;
-        u[m] = s_x[(wv * 64 + j + 4 * m) * CT + swb];
+        u[m] = s_x[(kk * 64 + j + 4 * m) * CT + swb];
       });
       DFT<16, -1>::run(u);
       __builtin_amdgcn_sched_barrier(0);
@@ -1065,7 +1120,7 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       __syncthreads();  // earlier users of the exchange buffer are done
       static_for<0, 16>([&](auto K) {
         constexpr int m = decltype(K)::value;
-        s_x[(wv * 64 + j + 4 * m) * CT + swb] = u[m];
+        s_x[(kk * 64 + j + 4 * m) * CT + swb] = u[m];
       });
       __syncthreads();
       static_for<0, 16>([&](auto K) {
@@ -1074,7 +1129,8 @@ __global__ __launch_bounds__(1024) void k_col_pass_x(ColArgs a) {
       });
     });
     __builtin_amdgcn_sched_barrier(0);
-    twiddle_a<M, +1>(v, s_tw, opaque(n1));
+    if constexpr (TWF) twiddle_a_row<M, +1>(v, s_tw + opaque(n1) * (M + 1));
+    else twiddle_a<M, +1>(v, s_tw, opaque(n1));
     DFT<M, +1>::run(v);
     if (ca < ncol) {
       static_for<0, M>([&](auto I) {
@@ -1273,7 +1329,7 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   using CF = Cfg<N>;
   if (h.panel && !CF::PANEL) return hipErrorInvalidValue;
   if constexpr (CF::XL) {
-    dim3 g((h.nx + 15) / 16, nwave), b(1024);
+    dim3 g((h.nx + CF::XCT - 1) / CF::XCT, nwave), b(64 * CF::XCT);
     if (!h.panel) launch_col_x<N, CF, false>(g, b, init, exit_, h.detect, a, st);
     else if constexpr (CF::PANEL) launch_col_x<N, CF, true>(g, b, init, exit_, h.detect, a, st);
   } else if constexpr (CF::WL) {

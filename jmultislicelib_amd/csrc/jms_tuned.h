@@ -7,7 +7,9 @@
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
 // default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
 // none), PADR (the same for the row pass), PANEL (waves of staged batches in
-// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly).  Included by jms_cfg.hpp after the plans.
+// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly);
+// at N = 1024, 2048 CW is the long-column kernel's columns per block (16 or
+// 8).  Included by jms_cfg.hpp after the plans.
 #pragma once
 
 namespace jms {
@@ -234,7 +236,7 @@ template <> struct Tuned<768> {
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
   static constexpr bool GTW = true;
-  static constexpr int CW = 16;
+  static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;

@@ -122,6 +122,7 @@ def choose(results):
         by.setdefault(r["n"], []).append(r)
     out = {}
     for n, rs in sorted(by.items()):
+        k = {}
         tuned = {r["variant"]: r for r in rs if r["pass"] == "row" and r["variant"].startswith("tuned")}
         if "tuned" in tuned and "tuned_flat" in tuned and tuned["tuned_flat"]["ms"] < 0.98 * tuned["tuned"]["ms"]:
             k["padr"] = 24
@@ -130,7 +131,6 @@ def choose(results):
             k["panel"] = True
         rows = [r for r in rs if r["pass"] == "row" and not r["variant"].startswith("tuned")]
         cols = [r for r in rs if r["pass"] == "col"]
-        k = {}
         if rows:
             d = next(r for r in rows if r["variant"] == "default" and r["gtw"] == 0)
             b = min(rows, key=lambda r: r["ms"])
@@ -144,6 +144,12 @@ def choose(results):
             b = min(cols, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
                 k["cw"], k["wpe"], k["tw"], k["pad"] = b["cw"], b["wpe"], b.get("tw", 0), b.get("pad", 3)
+        # long columns: 16 columns per block in the hardware order, or 8 in
+        # the block order jms_cfg.hpp gives them (pairs of waves, 16-tile rotation)
+        xcols = {r["ct"]: r for r in rs if r["pass"] == "xcol" and (r["ct"], r["g"], r["rot"], r["twf"]) in
+                 ((16, 0, 0, 0), (8, 2, 16, 0))}
+        if 16 in xcols and 8 in xcols and xcols[8]["ms"] < 0.98 * xcols[16]["ms"]:
+            k["cw"] = 8
         plans = [r for r in rs if r["pass"] == "plan"]
         if plans:  # the row and the column transforms each take their fastest plan
             dname = ",".join(map(str, default_plan(n)))
@@ -168,8 +174,10 @@ def emit(results, path):
              "// wave-local column block), WPE (waves per SIMD asked of the register",
              "// allocator for that kernel), TW (threads per column of that kernel; 0 =",
              "// default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20",
-             "// none), PADR (the same for the row pass).  Included by jms_cfg.hpp after",
-             "// the plans.",
+             "// none), PADR (the same for the row pass), PANEL (waves of staged batches in",
+             "// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly);",
+             "// at N = 1024, 2048 CW is the long-column kernel's columns per block (16 or",
+             "// 8).  Included by jms_cfg.hpp after the plans.",
              "#pragma once", "", "namespace jms {", "",
              "// FFT plan (radix order) per length: PlanFor<N> unless the tuner chose another",
              "template <int N>",
@@ -200,7 +208,7 @@ def emit(results, path):
         if "plan" in k:
             lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
-        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad", "padr"} & set(k):
+        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad", "padr", "panel", "tr"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")

@@ -54,6 +54,17 @@ struct ColV : B {
     }                                                                          \
   } while (0)
 
+// long-column kernel: CT columns per block, block order (groups of G waves
+// rotated by ROT tiles; G = 0 the hardware order), full twiddle table TWF
+template <int CT, int G, int ROT, bool TWF = false>
+struct XcolV : B {
+  static constexpr int XCT = CT;
+  static constexpr int CC = CT;
+  static constexpr int XG_X = G;
+  static constexpr int XROT_X = ROT;
+  static constexpr bool XTWF = TWF;
+};
+
 struct Bufs {
   float2 *w, *pro, *tw, *pg;
   int* var;
@@ -89,8 +100,8 @@ float time_col(const Bufs& b, int reps) {
   (void)hipEventRecord(e0);
   for (int i = 0; i < reps; ++i) {
     if constexpr (CF::XL) {
-      hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL>), dim3((N + 15) / 16, b.nw), dim3(1024), 0,
-                         0, a);
+      hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL>), dim3((N + CF::XCT - 1) / CF::XCT, b.nw),
+                         dim3(64 * CF::XCT), 0, 0, a);
     } else if constexpr (CF::WL) {
       hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, false, PANEL>), dim3((N + CF::CW - 1) / CF::CW, b.nw),
                          dim3(CF::TW * CF::CW), 0, 0, a);
@@ -237,6 +248,33 @@ int main() {
       col_variant<ColV<8, 1, 128>>(b, 8, 1, bytes, 128);
       col_variant<ColV<8, 8, 128>>(b, 8, 8, bytes, 128);
     }
+  }
+  if constexpr (B::XL) {
+    // 16 or 8 columns per block (8: two blocks per CU); hardware order,
+    // XCD-contiguous with block columns fastest, and that with each block
+    // row rotated by 8 block columns
+    auto xcol = [&](auto cf) {
+      using CF = decltype(cf);
+      float best = 1e30f;
+      for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
+      std::printf("{\"n\": %d, \"pass\": \"xcol\", \"ct\": %d, \"g\": %d, \"rot\": %d, \"twf\": %d, "
+                  "\"ms\": %.5f, \"gbs\": %.1f}\n", N, CF::XCT, CF::XG_X, CF::XROT_X, (int)CF::XTWF, best,
+                  bytes / (best * 1e-3) / 1e9);
+    };
+    xcol(XcolV<16, 0, 0>{});
+    xcol(XcolV<16, 2, 16>{});
+    xcol(XcolV<16, 2, 8>{});
+    xcol(XcolV<16, 2, 32>{});
+    xcol(XcolV<8, 0, 0>{});
+    xcol(XcolV<8, 1, 8>{});
+    xcol(XcolV<8, 1, 32>{});
+    xcol(XcolV<8, 2, 8>{});
+    xcol(XcolV<8, 2, 16>{});
+    xcol(XcolV<8, 2, 32>{});
+    xcol(XcolV<8, 4, 8>{});
+    xcol(XcolV<8, 4, 16>{});
+    xcol(XcolV<8, 4, 32>{});
+    xcol(XcolV<8, 8, 16>{});
   }
 #ifdef JMS_PLANS_FILE
 #define JMS_PLAN_CASE(NAME, ...) plan_variant<__VA_ARGS__>(b, NAME, bytes);

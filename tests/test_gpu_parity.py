@@ -246,6 +246,26 @@ def test_full_2048_grid_batch_vs_oracle():
     eng.close()
 
 
+def test_full_1024_grid_batch_vs_oracle():
+    """1024^2 (the long-column kernel with 8-column tiles in pairs of waves):
+    a staged batch of three probe passes, the last pair of waves short, with
+    detection planes every 2 slices, against the oracle."""
+    sp = random_setup(1024, 1024, nslices=5, nvar=(2, 2), ndetper=2, seed=23)
+    npos = 3
+    dx = np.array([0.3, 1.1, 0.55], np.float32)
+    dy = np.array([0.7, 0.2, 0.95], np.float32)
+    var = configs.variant_sequences(sp, npos, seed=5)
+    eng = Engine().setup(sp)
+    eng.set_batch(npos)
+    eng.stage_scan(dx, dy, None, np.ones(1, np.float32), var)
+    eng.run_staged(0, npos)
+    eng.sync()
+    g = eng.scan_results()
+    ref = O.stem_scan(sp, dx, dy, np.ones(1, np.float32), var, prec=32, nthreads=3)
+    assert_dets(g, ref, TOL, "1024^2 batch")
+    eng.close()
+
+
 def test_reference_sample_job_shape_625():
     """The shape of the reference's own sample job (test/JMSBench1/test with
     STO_test.slp): 625^2 pixels over 4x4 SrTiO3 cells at 300 kV, 64 slices
