@@ -940,10 +940,12 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
   const int nx = a.nx;
   const int x0 = bx * CT;
   const int ncol = (nx - x0 < CT) ? (nx - x0) : CT;
-  // PANEL: the tile is one contiguous panel (element (y, x) at x0 N + 16 y + x - x0)
-  static_assert(!PANEL || CT == kPanel, "a panel is one tile");
+  // PANEL: the tile lies in one panel (element (y, x) at p N + 16 y + x - p,
+  // p = x0 rounded down to a multiple of 16)
+  static_assert(!PANEL || kPanel % CT == 0, "a tile lies in one panel");
   const int wrs = PANEL ? kPanel : nx;  // row stride of the wave
-  float2* wbase = a.wave + w * a.wave_stride + (PANEL ? (long long)x0 * N : x0);
+  float2* wbase =
+      a.wave + w * a.wave_stride + (PANEL ? (long long)(x0 & ~(kPanel - 1)) * N + (x0 & (kPanel - 1)) : x0);
   // A layout (columns past the grid edge work on column 0 of the tile and
   // are not stored)
   const int ca = tid % CT, n1 = tid / CT;
