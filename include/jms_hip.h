@@ -148,14 +148,15 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
  * together: one row-pass and one column-pass launch per slice step over the
  * waves of the members that still have slices (plus one column launch for
  * the members whose exit plane it is), driven by per-wave, per-member and
- * per-plane descriptor tables.  Enqueued on the first member's stream (in
- * the order given); every member's stream waits for the group, so
- * jms_hip_sync on any member waits for it.  Members with more positions
+ * per-plane descriptor tables.  Enqueued on the stream of the member with
+ * the most slices (the first such in the order given), after the work
+ * already on every member's stream; every member's stream waits for the
+ * group, so jms_hip_sync on any member waits for it.  Members with more positions
  * than one batch (jms_hip_set_batch) continue in further rounds.  Integrated
  * detectors only; grids whose column pass is the wave-local kernel (edges up
  * to 768).  create: 2 invalid list, 3 an engine twice, 4 grids or devices
- * differ (or no grid), 5 grid not supported.  Destroy a group before its
- * members. */
+ * differ (or no grid), 5 grid not supported; run: 18 a member's grid or
+ * device changed since create.  Destroy a group before its members. */
 typedef struct jms_hip_group jms_hip_group;
 int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, int n);
 int jms_hip_group_run(jms_hip_group* g);

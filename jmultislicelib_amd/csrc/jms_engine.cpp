@@ -1157,6 +1157,8 @@ const char* jms_hip_group_last_error(jms_hip_group* g) { return g ? g->err.c_str
 int jms_hip_group_run(jms_hip_group* g) {
   if (!g) return 1;
   for (auto* e : g->eng) {
+    if (e->nx != g->nx || e->ny != g->ny || e->device != g->device)
+      return g->fail(18, "member grid or device changed since the group was made");
     int rc = check_ready(e);
     if (rc) return g->fail(10 + rc, "member not ready");
     if (!e->wav0.p) return g->fail(16, "member probe not set");

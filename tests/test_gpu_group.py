@@ -111,5 +111,11 @@ def test_group_refuses_mixed_or_unsupported_grids():
     with pytest.raises(JMSError):
         g.run()
     g.close()
+    g = Group([b])  # a member's grid changed after the group was made
+    b._c("jms_hip_set_grid", 360, 360, 2.0, 2.0, 0.0025)
+    with pytest.raises(JMSError) as ex:
+        g.run()
+    assert ex.value.code == 18
+    g.close()
     for e in (a, b, c):
         e.close()
