@@ -596,9 +596,12 @@ int CJMultiSlice::PhaseGratingSetup(int whichcode, int nx, int ny, int nslc, int
 int CJMultiSlice::ObjectSliceSetup(int nobjslc, int* objslc) {
   m_nobjslc = 0;
   m_objslc.clear();
-  if (nobjslc < 1 || !objslc) return 0;
-  m_nobjslc = nobjslc;
-  m_objslc.assign(objslc, objslc + nobjslc);
+  if (nobjslc >= 1 && objslc) {
+    m_nobjslc = nobjslc;
+    m_objslc.assign(objslc, objslc + nobjslc);
+  }
+  // no object slices is a valid set-up (JMultiSlice.cpp:528-537): the
+  // multislice then reads the incident wave out at the entrance plane
   m_status_setup_GPU |= _JMS_STATUS_OBJ;
   return 0;
 }

@@ -120,6 +120,42 @@ def test_class_api_dropin_vs_oracle():
     J.Cleanup()
 
 
+def test_zero_object_slices_read_the_incident_wave():
+    """No object slices (ObjectSliceSetup with an empty list, a valid set-up in
+    the reference): the exit plane is the entrance plane, so each detector
+    reads sum_mask |probe|^2 * det of the incident probe (a lateral offset is
+    a phase ramp and leaves |psi(k)|^2 unchanged); checked per detector
+    against that sum in float64."""
+    sp = configs.perovskite_setup(n=64, ncell=2, nslices=10, nvar=3)
+    J = JMultiSlice()
+    J.Cleanup()
+    J.SetHighTension(sp.ht)
+    J.SetGridSize(sp.nx, sp.ny)
+    J.SetSupercellSize(*sp.a0)
+    assert J.PhaseGratingSetup(GPU, sp.nx, sp.ny, 2, 3, [3, 3]) == 0
+    assert J.ObjectSliceSetup([]) == 0
+    assert J.PropagatorSetup(GPU, 1, [0, 0]) == 0
+    assert J.DetectorSetup(GPU, 0, 3, 1, 1) == 0
+    for i, g in enumerate(sp.gratings):
+        assert J.SetPhaseGratingData(GPU, i, g) == 0
+    assert J.SetPropagatorData(GPU, 0, sp.propagators[0]) == 0
+    for i, (d, m) in enumerate(sp.detectors):
+        assert J.SetDetectorData(GPU, i, d, m) == 0
+    assert J.InitCore(GPU, 1) == 0
+    assert J.SetIncidentWave(GPU, sp.probe) == 0
+    p2 = np.abs(sp.probe.astype(np.complex128).ravel()) ** 2
+    want = np.array([np.sum(p2[np.asarray(m)] * np.asarray(d, np.float64).ravel()[np.asarray(m)])
+                     for d, m in sp.detectors])
+    assert want[0] > 0.1
+    for dx, dy in ((0.0, 0.0), (0.33, 0.12)):
+        assert J.OffsetIncomingWave(GPU, dx, dy, 0.0) == 0
+        assert J.GPUMultislice(0, 0, 1.0) == 0
+        got = J.GetResult(GPU, 1)[0]
+        err = np.abs(got - want) / np.maximum(np.abs(want), 1e-12)
+        assert np.all(err < 1e-5), (got, want)
+    J.Cleanup()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_staged_scan_vs_oracle(graph):
     sp = configs.perovskite_setup(n=64, ncell=2, nslices=16, nvar=4)
