@@ -308,6 +308,17 @@ int enqueue_batch(jms_hip_engine* e, void* waves, float* partial, int nwave, int
     }
   }
   if (npass > 0 && res && ca.ndet > 0) {
+    // detection planes before islc0 are not read out in this batch: their
+    // partial sums hold nothing of it (GPUMultislice reads only the planes
+    // from islc0 on, JMultiSlice.cpp:4568-4590)
+    for (int p = 0; p < islc0 && p <= S; ++p) {
+      const int id = e->det_objslc[p];
+      bool later = false;
+      for (int q = islc0; q <= S && !later; ++q) later = e->det_objslc[q] == id;
+      if (id < 0 || later) continue;
+      err = jms::launch_zero_partial(partial, part_w, (long long)id * e->ndet * ntc, e->ndet * ntc, nwave, e->st);
+      if (err != hipSuccess) return e->fail(134, "clear unread planes", err);
+    }
     TimedLaunch tl(e, 2, timed);
     err = jms::launch_combine(partial, ntc, e->ndet, e->ndetslc, npos, npass, weights,
                               res, accumulate, e->st);

@@ -154,6 +154,10 @@ hipError_t launch_col_pass(int ny, int nwave, bool init, bool exit_, const ColAr
                            hipStream_t st);
 hipError_t launch_fft2d(int nx, int ny, int nbatch, float2* data, int dir, const float2* twx,
                         const float2* twy, hipStream_t st);
+// zero partial[w * part_w + off + k], k < len, for the nwave waves (the
+// partial sums of a detection plane a batch does not read out)
+hipError_t launch_zero_partial(float* partial, long long part_w, long long off, int len, int nwave,
+                               hipStream_t st);
 hipError_t launch_combine(const float* partial, int ntiles, int ndet, int ndetslc, int npos,
                           int npass, const float* weights, float* out, int accumulate,
                           hipStream_t st);

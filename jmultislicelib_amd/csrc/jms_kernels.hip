@@ -61,6 +61,11 @@ __global__ void k_combine(const float* __restrict__ partial, int ntiles, int nde
   out[i] = accumulate ? (float)((double)out[i] + acc) : (float)acc;
 }
 
+__global__ void k_zero_partial(float* __restrict__ partial, long long part_w, long long off, int len) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < len) partial[blockIdx.y * part_w + off + k] = 0.f;
+}
+
 // psi_out = psi0 * exp(-i [2 pi (kx dx + ky dy + k^2 fz) + cppl]) in Fourier space
 // (OffsetIncomingWave GPU branch, JMultiSlice.cpp:2036-2077; ArrayOps.cu:390-422),
 // with an optional cyclic beam-tilt shift of psi0 by (ish0, ish1) pixels
@@ -468,6 +473,13 @@ hipError_t launch_combine(const float* partial, int ntiles, int ndet, int ndetsl
   if (total <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_combine, dim3((total + 255) / 256), dim3(256), 0, st, partial, ntiles, ndet,
                      ndetslc, npos, npass, weights, out, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_partial(float* partial, long long part_w, long long off, int len, int nwave,
+                               hipStream_t st) {
+  if (len <= 0 || nwave <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_partial, dim3((len + 255) / 256, nwave), dim3(256), 0, st, partial, part_w, off, len);
   return hipGetLastError();
 }
 

@@ -120,6 +120,39 @@ def test_class_api_dropin_vs_oracle():
     J.Cleanup()
 
 
+@pytest.mark.parametrize("islc0", [3, 6])
+def test_start_inside_the_stack_vs_oracle(islc0):
+    """GPUMultislice(islc0 > 0) (JMultiSlice.cpp:4568-4590): the incident wave
+    enters at plane islc0 and only the planes from there on are read out;
+    single-wave path and a staged batch against the oracle started at the
+    same plane (islc0 = 6 is the exit plane: a readout only)."""
+    sp = random_setup(96, 64, nslices=6, nvar=(3, 2), ndetper=2, seed=31)
+    var = configs.variant_sequences(sp, 2, seed=6)
+    eng = Engine().setup(sp)
+    win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.21, 0.13)
+    eng.wave_offset(0.21, 0.13, 0.0)
+    eng.multislice(var[0], 1.0, False, islc0=islc0)
+    g = eng.result(1)
+    ref, _ = O.multislice(sp, win, var[0], 1.0, islc0=islc0)
+    assert_dets(g, ref, TOL, f"single wave from plane {islc0}")
+    assert np.count_nonzero(ref) > 0
+    dx = np.array([0.21, 0.5], np.float32)
+    dy = np.array([0.13, 0.02], np.float32)
+    eng.set_batch(2)
+    eng.stage_scan(dx, dy, None, np.ones(1, np.float32), var)
+    eng.run_staged(0, 2)  # a full run first: its readouts of the early planes must not leak
+    eng.sync()
+    assert np.count_nonzero(eng.scan_results()[:, 0]) > 0
+    eng.run_staged(0, 2, islc0)
+    eng.sync()
+    gs = eng.scan_results()
+    for p in range(2):
+        w_p = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, float(dx[p]), float(dy[p]))
+        r_p, _ = O.multislice(sp, w_p, var[p], 1.0, islc0=islc0)
+        assert_dets(gs[p], r_p, TOL, f"staged position {p} from plane {islc0}")
+    eng.close()
+
+
 def test_zero_object_slices_read_the_incident_wave():
     """No object slices (ObjectSliceSetup with an empty list, a valid set-up in
     the reference): the exit plane is the entrance plane, so each detector
