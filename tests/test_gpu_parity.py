@@ -153,6 +153,37 @@ def test_start_inside_the_stack_vs_oracle(islc0):
     eng.close()
 
 
+def test_detectors_without_masks_vs_oracle():
+    """Detectors uploaded without a pixel mask (msklen 0) weight every pixel
+    (the reference's CPU readout, DotProduct_h): a smooth random weight map
+    and a ring detector with its mask dropped, beside a masked ring; single
+    wave and staged batch against the oracle."""
+    sp = random_setup(64, 96, nslices=5, nvar=(2, 3), ndetper=2, seed=41)
+    rng = np.random.default_rng(5)
+    ky, kx = np.meshgrid(np.fft.fftfreq(sp.ny), np.fft.fftfreq(sp.nx), indexing="ij")
+    smooth = (0.5 + 0.5 * np.cos(6.0 * kx) * np.cos(4.0 * ky) + 0.1 * rng.random((sp.ny, sp.nx))).astype(np.float32)
+    d0, m0 = sp.detectors[0]
+    d2, _ = sp.detectors[2]
+    sp.detectors = [(d0, m0), (smooth, None), (d2, None)]
+    var = configs.variant_sequences(sp, 2, seed=8)
+    eng = Engine().setup(sp)
+    win = O.offset_wave(sp.probe, sp.a0[0], sp.a0[1], sp.wl, 0.17, 0.29)
+    eng.wave_offset(0.17, 0.29, 0.0)
+    eng.multislice(var[0], 1.0, False)
+    ref, _ = O.multislice(sp, win, var[0], 1.0)
+    assert_dets(eng.result(1), ref, TOL, "single wave, mask-less detectors")
+    assert ref[-1, 1] > 0.5  # the weight map sees most of the intensity
+    dx = np.array([0.17, 0.6], np.float32)
+    dy = np.array([0.29, 0.05], np.float32)
+    eng.set_batch(2)
+    eng.stage_scan(dx, dy, None, np.ones(1, np.float32), var)
+    eng.run_staged(0, 2)
+    eng.sync()
+    ref_s = O.stem_scan(sp, dx, dy, np.ones(1, np.float32), var, prec=32, nthreads=2)
+    assert_dets(eng.scan_results(), ref_s, TOL, "staged, mask-less detectors")
+    eng.close()
+
+
 def test_zero_object_slices_read_the_incident_wave():
     """No object slices (ObjectSliceSetup with an empty list, a valid set-up in
     the reference): the exit plane is the entrance plane, so each detector
