@@ -290,7 +290,9 @@ def test_graph_cache_follows_batch_shape_and_buffers():
 @pytest.mark.parametrize("nx,ny", [(128, 64), (64, 128), (360, 384), (96, 160), (2048, 64), (64, 2048),
                                    (1024, 96), (96, 1024), (768, 640), (360, 1024), (48, 2048),
                                    (384, 360), (96, 480), (480, 720), (720, 64), (320, 48), (48, 320),
-                                   (32, 128), (625, 625), (625, 96), (64, 625)])
+                                   (32, 128), (625, 625), (625, 96), (64, 625),
+                                   # long columns with a partial last tile (1 or 8 columns)
+                                   (625, 1024), (625, 2048), (360, 2048)])
 def test_nonsquare_mixed_radix(nx, ny):
     sp = random_setup(nx, ny, nslices=6, nvar=(3, 2), ndetper=2, seed=nx + ny)
     var = configs.variant_sequences(sp, 2, seed=3)
