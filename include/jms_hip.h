@@ -156,7 +156,9 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
  * detectors only; grids whose column pass is the wave-local kernel (edges up
  * to 768).  create: 2 invalid list, 3 an engine twice, 4 grids or devices
  * differ (or no grid), 5 grid not supported; run: 18 a member's grid or
- * device changed since create.  Destroy a group before its members. */
+ * device changed since create, 19 a member was destroyed (the group stays
+ * valid to destroy).  Destroying a group waits for its work on every live
+ * member's stream. */
 typedef struct jms_hip_group jms_hip_group;
 int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, int n);
 int jms_hip_group_run(jms_hip_group* g);

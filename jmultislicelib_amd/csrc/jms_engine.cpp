@@ -54,10 +54,14 @@ struct Timer {
 
 }  // namespace
 
+// a member of group g is being destroyed (defined with the grouped launches)
+void group_member_gone(jms_hip_group* g, jms_hip_engine* e);
+
 struct jms_hip_engine {
   int device = 0;
   hipStream_t st = nullptr;
   std::string err;
+  std::vector<jms_hip_group*> groups;  // grouped launches this engine is a member of
   // grid
   int nx = 0, ny = 0;
   long long n = 0;
@@ -405,6 +409,8 @@ void jms_hip_destroy(jms_hip_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->st) (void)hipStreamSynchronize(e->st);
+  for (jms_hip_group* g : e->groups) group_member_gone(g, e);  // those groups refuse to run from now on
+  e->groups.clear();
   e->drop_graphs();
   for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
   DevBuf* bufs[] = {&e->twx, &e->twy, &e->pgr, &e->pro, &e->pro_t, &e->detw, &e->detw_t, &e->wav0,
@@ -984,6 +990,7 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
 struct jms_hip_group {
   std::vector<jms_hip_engine*> eng;  // members, by slice count descending
   int device = 0, nx = 0, ny = 0, smax = 0;
+  bool orphaned = false;             // a member was destroyed: the group cannot run
   std::string err;
   struct Round {
     size_t job_off = 0, wav_off = 0;  // byte offsets in tab
@@ -1006,6 +1013,12 @@ struct jms_hip_group {
     return code;
   }
 };
+
+void group_member_gone(jms_hip_group* g, jms_hip_engine* e) {
+  for (auto& m : g->eng)
+    if (m == e) m = nullptr;
+  g->orphaned = true;
+}
 
 namespace {
 
@@ -1134,19 +1147,21 @@ int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, in
   if (!jms::grp_supported(e0->nx, e0->ny)) return 5;
   jms_hip_group* g = new jms_hip_group();
   g->eng.assign(engines, engines + n);
+  for (jms_hip_engine* e : g->eng) e->groups.push_back(g);
   std::stable_sort(g->eng.begin(), g->eng.end(),
                    [](const jms_hip_engine* a, const jms_hip_engine* b) { return a->nobjslc > b->nobjslc; });
   g->device = e0->device;
   g->nx = e0->nx;
   g->ny = e0->ny;
   (void)hipSetDevice(g->device);
-  if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess) { delete g; return 101; }
+  if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess) {
+    jms_hip_group_destroy(g);
+    return 101;
+  }
   g->ev_in.resize(n, nullptr);
   for (int k = 0; k < n; ++k)
     if (hipEventCreateWithFlags(&g->ev_in[k], hipEventDisableTiming) != hipSuccess) {
-      for (auto ev : g->ev_in) if (ev) (void)hipEventDestroy(ev);
-      (void)hipEventDestroy(g->ev_done);
-      delete g;
+      jms_hip_group_destroy(g);
       return 101;
     }
   *out = g;
@@ -1156,7 +1171,11 @@ int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, in
 void jms_hip_group_destroy(jms_hip_group* g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
-  if (!g->eng.empty()) (void)hipStreamSynchronize(g->eng[0]->st);
+  for (jms_hip_engine* e : g->eng) {  // live members: wait for the group's work, forget the group
+    if (!e) continue;
+    (void)hipStreamSynchronize(e->st);
+    e->groups.erase(std::remove(e->groups.begin(), e->groups.end(), g), e->groups.end());
+  }
   g->tab.release();
   for (auto ev : g->ev_in) if (ev) (void)hipEventDestroy(ev);
   if (g->ev_done) (void)hipEventDestroy(g->ev_done);
@@ -1167,6 +1186,7 @@ const char* jms_hip_group_last_error(jms_hip_group* g) { return g ? g->err.c_str
 
 int jms_hip_group_run(jms_hip_group* g) {
   if (!g) return 1;
+  if (g->orphaned) return g->fail(19, "a member engine was destroyed");
   for (auto* e : g->eng) {
     if (e->nx != g->nx || e->ny != g->ny || e->device != g->device)
       return g->fail(18, "member grid or device changed since the group was made");

@@ -111,6 +111,17 @@ def test_group_refuses_mixed_or_unsupported_grids():
     with pytest.raises(JMSError):
         g.run()
     g.close()
+    d = Engine()  # a member destroyed before the group: the group refuses to run, then closes cleanly
+    e2 = Engine()
+    d._c("jms_hip_set_grid", 360, 360, 2.0, 2.0, 0.0025)
+    e2._c("jms_hip_set_grid", 360, 360, 2.0, 2.0, 0.0025)
+    g = Group([d, e2])
+    d.close()
+    with pytest.raises(JMSError) as ex:
+        g.run()
+    assert ex.value.code == 19
+    g.close()
+    e2.close()
     g = Group([b])  # a member's grid changed after the group was made
     b._c("jms_hip_set_grid", 360, 360, 2.0, 2.0, 0.0025)
     with pytest.raises(JMSError) as ex:
