@@ -151,13 +151,13 @@ int jms_hip_run_batch_noalloc(jms_hip_engine* e, int islc0, int nwave, void* wav
  * per-plane descriptor tables.  Enqueued on the stream of the member with
  * the most slices (the first such in the order given), after the work
  * already on every member's stream; every member's stream waits for the
- * group, so jms_hip_sync on any member waits for it.  Members with more positions
- * than one batch (jms_hip_set_batch) continue in further rounds.  Integrated
- * detectors only; grids whose column pass is the wave-local kernel (edges up
- * to 768).  create: 2 invalid list, 3 an engine twice, 4 grids or devices
- * differ (or no grid), 5 grid not supported; run: 18 a member's grid or
- * device changed since create, 19 a member was destroyed (the group stays
- * valid to destroy).  Destroying a group waits for its work on every live
+ * group, so jms_hip_sync on any member waits for it.  Members with more
+ * positions than one batch (jms_hip_set_batch) continue in further rounds.
+ * Integrated detectors only, every supported grid.  create: 2 invalid list,
+ * 3 an engine twice, 4 grids or devices differ (or no grid), 5 grid not
+ * supported; run: 18 a member's grid or device changed since create, 19 a
+ * member was destroyed (the group stays valid to destroy), 20 a member has a
+ * diffraction de-scan.  Destroying a group waits for its work on every live
  * member's stream. */
 typedef struct jms_hip_group jms_hip_group;
 int jms_hip_group_create(jms_hip_group** out, jms_hip_engine* const* engines, int n);

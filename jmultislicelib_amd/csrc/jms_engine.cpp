@@ -1052,6 +1052,7 @@ int group_build(jms_hip_group* g, std::vector<char>& host) {
         const int js = e->objslc[p] % e->nscslc;
         q.slc_off = e->slcoffset[js];
         q.pro_t = e->pro_t.as<float2>() + (long long)e->proidx[js] * e->n;
+        q.pro = e->pro.as<float2>() + (long long)e->proidx[js] * e->n;
       }
       const int id = e->det_objslc[p];
       q.part_off = (id >= 0 && ndet > 0) ? (long long)id * ndet * ntc : -1;
@@ -1085,6 +1086,8 @@ int group_build(jms_hip_group* g, std::vector<char>& host) {
       jb.pgr = e->pgr.p;
       jb.detw_t = e->detw_t.as<float>();
       jb.cppl_t = e->have_cppl ? e->cppl_t.as<float>() : nullptr;
+      jb.detw = e->detw.as<float>();
+      jb.cppl = e->have_cppl ? e->cppl.as<float>() : nullptr;
       jb.kscx = 1.f / e->a0x;
       jb.kscy = 1.f / e->a0y;
       jb.nslc = S;
@@ -1190,6 +1193,7 @@ int jms_hip_group_run(jms_hip_group* g) {
   for (auto* e : g->eng) {
     if (e->nx != g->nx || e->ny != g->ny || e->device != g->device)
       return g->fail(18, "member grid or device changed since the group was made");
+    if (e->dsx != 0 || e->dsy != 0) return g->fail(20, "member with a diffraction de-scan (not grouped)");
     int rc = check_ready(e);
     if (rc) return g->fail(10 + rc, "member not ready");
     if (!e->wav0.p) return g->fail(16, "member probe not set");

@@ -94,6 +94,8 @@ struct GrpJob {
   const void* pgr = nullptr;      // phase gratings (float2)
   const float* detw_t = nullptr;  // [ndet][nx][ny] detector weights
   const float* cppl_t = nullptr;  // constant phase plate [x][y] or nullptr
+  const float* detw = nullptr;    // the same row-major [ndet][ny][nx] (long-column kernel)
+  const float* cppl = nullptr;    // the same row-major [y][x] or nullptr
   float kscx = 0.f, kscy = 0.f;   // 1/a0x, 1/a0y [1/nm]
   int nslc = 0;                   // slices of the job (row length of var)
   int ndet = 0;
@@ -104,6 +106,7 @@ struct GrpJob {
 struct GrpPlane {
   long long slc_off = 0;          // grating offset of the slice of step p (variant 0)
   const void* pro_t = nullptr;    // propagator of step p, column-major (float2)
+  const void* pro = nullptr;      // the same row-major (long-column kernel)
   long long part_off = -1;        // offset of plane p in partial; -1: no readout
 };
 struct GrpWave {
@@ -118,7 +121,8 @@ struct GrpArgs {
   int w0 = 0;                     // first wave of the launch in the wave map
   int step = 0;                   // row pass: slice step; column pass: plane read out
 };
-// grouped launches exist for grids whose column pass is the wave-local kernel
+// grouped launches exist for grids whose column pass is the wave-local or
+// the long-column kernel
 bool grp_supported(int nx, int ny);
 // row pass of step ga.step over nwave waves of the map (from ga.w0)
 hipError_t launch_row_pass_grp(int nx, int ny, int nwave, long long wave_stride, long long npg,

@@ -407,7 +407,7 @@ bool panel_ok(int nx, int ny) {
 bool grp_supported(int nx, int ny) {
   if (!size_supported(nx)) return false;
   switch (ny) {
-#define JMS_CASE(N) case N: return Cfg<N>::WL && !Cfg<N>::XL;
+#define JMS_CASE(N) case N: return Cfg<N>::WL || Cfg<N>::XL;
     JMS_SIZES(JMS_CASE)
 #undef JMS_CASE
     default: return false;

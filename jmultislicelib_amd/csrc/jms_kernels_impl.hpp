@@ -909,8 +909,8 @@ __device__ __forceinline__ void twiddle_a_row(float2 (&v)[M], const float2* row)
   });
 }
 
-template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool PANEL = false>
-__global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4))) void k_col_pass_x(ColArgs a) {
+template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool PANEL = false, bool GRP = false>
+__global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4))) void k_col_pass_x(ColArgs a0) {
   constexpr int M = N / 64;
   static_assert(M == 16 || M == 32, "long-column kernel: N = 1024 or 2048");
   constexpr int R = M / 16;   // exchange rounds
@@ -928,15 +928,40 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
   __shared__ float s_red[16 * kMaxDet];
   const int tid = threadIdx.x;
   if constexpr (TWF) {
-    for (int e = tid; e < 64 * M; e += NT) s_tw[(e / M) * (M + 1) + e % M] = a.twg[(e / M) * (e % M)];
+    for (int e = tid; e < 64 * M; e += NT) s_tw[(e / M) * (M + 1) + e % M] = a0.twg[(e / M) * (e % M)];
   } else {
-    for (int e = tid; e < N / 2; e += NT) s_tw[e] = a.twg[e];
+    for (int e = tid; e < N / 2; e += NT) s_tw[e] = a0.twg[e];
   }
-  if (tid < 64) s_w64[tid] = a.twg[((tid & 3) * (tid >> 2) * M) % N];
+  if (tid < 64) s_w64[tid] = a0.twg[((tid & 3) * (tid >> 2) * M) % N];
   int bx, by;
   if constexpr (C_::XG_X > 0) block_xy_groups<C_::XG_X, C_::XROT_X>(bx, by);
   else block_xy<C_::XCD_COL>(bx, by);
-  const long long w = by;
+  // GRP: block row by is wave grp.w0 + by of the wave map; its job supplies
+  // the buffers, and the plane table the propagator and the readout of plane
+  // grp.step (each job reads out on its own planes)
+  ColArgs a = a0;
+  long long w = by;
+  if constexpr (GRP) {
+    const GrpWave gw = a0.grp.wav[a0.grp.w0 + by];
+    const GrpJob& jb = a0.grp.job[gw.job];
+    const GrpPlane* pl = a0.grp.pl + (long long)gw.job * a0.grp.pl_ld;
+    w = gw.w;
+    a.wave = (float2*)jb.wave;
+    a.src = (const float2*)jb.src;
+    a.src_stride = 0;
+    a.ppar = (const float4*)jb.ppar;
+    a.kscx = jb.kscx;
+    a.kscy = jb.kscy;
+    a.cppl = jb.cppl;
+    a.ndet = jb.ndet;
+    a.detw = jb.detw;
+    a.partial = jb.partial;
+    a.part_wstride = jb.part_wstride;
+    const long long po = pl[a0.grp.step].part_off;
+    a.detect = po >= 0 ? 1 : 0;
+    a.part_off = po >= 0 ? po : 0;
+    if constexpr (!INIT) a.pro = (const float2*)pl[a0.grp.step - 1].pro;
+  }
   const int nx = a.nx;
   const int x0 = bx * CT;
   const int ncol = (nx - x0 < CT) ? (nx - x0) : CT;
@@ -1049,7 +1074,7 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
       });
     }
     if constexpr (DET) {
-      if (validb) {
+      if (validb && a.detect) {
         // de-scan (DescanDifN, JMultiSlice.cpp:2184-2275): pixel i is
         // weighted with det(i - d)
         const int xs = wrap_sub(xb, a.dsx, nx);
@@ -1100,8 +1125,10 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
       }
     }
   });
-  if constexpr (DET)
-    block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + bx, a.part_dstride);
+  if constexpr (DET) {
+    if (a.detect)  // uniform over the block (one wave)
+      block_reduce_det(acc, a.ndet, s_red, a.partial + w * a.part_wstride + a.part_off + bx, a.part_dstride);
+  }
   if constexpr (!EXIT) {
     static_for<0, R>([&](auto RR) {
       constexpr int r = decltype(RR)::value;
@@ -1296,18 +1323,18 @@ void launch_col_w(dim3 g, dim3 b, bool init, bool exit_, bool detect, const ColA
     else hipLaunchKernelGGL((k_col_pass_w<N, false, false, false, CF, GRP, PANEL>), g, b, 0, st, a);
   }
 }
-template <int N, class CF, bool PANEL>
+template <int N, class CF, bool PANEL, bool GRP = false>
 void launch_col_x(dim3 g, dim3 b, bool init, bool exit_, bool detect, const ColArgs& a, hipStream_t st) {
   if (detect) {
-    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, true, CF, PANEL>), g, b, 0, st, a);
-    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, true, CF, PANEL>), g, b, 0, st, a);
-    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, true, CF, PANEL>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, true, CF, PANEL>), g, b, 0, st, a);
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, true, CF, PANEL, GRP>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, true, CF, PANEL, GRP>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, true, CF, PANEL, GRP>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, true, CF, PANEL, GRP>), g, b, 0, st, a);
   } else {
-    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, false, CF, PANEL>), g, b, 0, st, a);
-    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, false, CF, PANEL>), g, b, 0, st, a);
-    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, false, CF, PANEL>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL>), g, b, 0, st, a);
+    if (init && exit_) hipLaunchKernelGGL((k_col_pass_x<N, true, true, false, CF, PANEL, GRP>), g, b, 0, st, a);
+    else if (init) hipLaunchKernelGGL((k_col_pass_x<N, true, false, false, CF, PANEL, GRP>), g, b, 0, st, a);
+    else if (exit_) hipLaunchKernelGGL((k_col_pass_x<N, false, true, false, CF, PANEL, GRP>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_col_pass_x<N, false, false, false, CF, PANEL, GRP>), g, b, 0, st, a);
   }
 }
 
@@ -1349,12 +1376,24 @@ hipError_t launch_col_pass_n(int nwave, bool init, bool exit_, const ColArgsHost
   return hipGetLastError();
 }
 
-// grouped column pass (waves of several jobs; the wave-local kernel only)
+// grouped column pass (waves of several jobs; the wave-local and the
+// long-column kernel)
 template <int N>
 hipError_t launch_col_pass_grp_n(int nx, int nwave, bool init, bool exit_, bool detect, long long part_dstride,
                                  const float2* twy, const GrpArgs& ga, bool panel, hipStream_t st) {
   using CF = Cfg<N>;
-  if constexpr (CF::WL && !CF::XL) {
+  if constexpr (CF::XL) {
+    if (panel) return hipErrorInvalidValue;
+    ColArgs a{};
+    a.nx = nx;
+    a.wave_stride = (long long)N * nx;
+    a.part_dstride = part_dstride;
+    a.twg = twy;
+    a.grp = ga;
+    dim3 g((nx + CF::XCT - 1) / CF::XCT, nwave), b(64 * CF::XCT);
+    launch_col_x<N, CF, false, true>(g, b, init, exit_, detect, a, st);
+    return hipGetLastError();
+  } else if constexpr (CF::WL) {
     if (panel && !CF::PANEL) return hipErrorInvalidValue;
     ColArgs a{};
     a.nx = nx;

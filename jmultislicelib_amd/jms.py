@@ -466,7 +466,7 @@ class Group:
     Engine.run_staged; Engine.sync on any member waits for the group."""
 
     _CREATE_ERRORS = {2: "invalid engine list", 3: "an engine given twice", 4: "grids or devices differ",
-                      5: "grid not supported by the grouped kernels (column pass not wave-local)"}
+                      5: "grid not supported by the grouped kernels"}
 
     def __init__(self, engines):
         self.L = lib()

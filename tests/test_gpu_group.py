@@ -14,7 +14,7 @@ import pytest
 from jmultislicelib_amd import Engine, Group, JMSError, configs
 from oracle import oracle as O
 
-from helpers import assert_dets_noise_aware
+from helpers import assert_dets_noise_aware, random_setup
 
 pytestmark = pytest.mark.gpu
 
@@ -49,7 +49,7 @@ def _engine(sp, dx, dy, w, var, batch=None):
     return eng
 
 
-@pytest.mark.parametrize("edge", [360, 512])
+@pytest.mark.parametrize("edge", [360, 512, 1024])
 def test_group_equals_separate_runs_and_oracle(edge):
     jobs = _jobs(edge)
     # member 2 gets a batch of two waves: it runs over several rounds
@@ -91,7 +91,41 @@ def test_group_equals_separate_runs_and_oracle(edge):
         e.close()
 
 
-def test_group_refuses_mixed_or_unsupported_grids():
+def test_group_long_columns_2048():
+    """2048^2 (the long-column kernel in its grouped form): three jobs with
+    different slice counts, positions and detection planes as one group; each
+    equals its own staged run to float32 rounding, and the smallest matches
+    the oracle."""
+    jobs = []
+    for k, (ns, npos, ndetper) in enumerate(((4, 2, 0), (2, 3, 1), (3, 1, 2))):
+        sp = random_setup(2048, 2048, nslices=ns, nvar=(2, 1), ndetper=ndetper, seed=50 + k)
+        dx = np.linspace(0.1, 0.9, npos).astype(np.float32)
+        dy = np.linspace(0.7, 0.2, npos).astype(np.float32)
+        jobs.append((sp, dx, dy, np.ones(1, np.float32), configs.variant_sequences(sp, npos, seed=k)))
+    grouped = [_engine(*j) for j in jobs]
+    alone = [_engine(*j) for j in jobs]
+    grp = Group(grouped)
+    grp.run()
+    grp.sync()
+    for eng, j in zip(alone, jobs):
+        eng.run_staged(0, len(j[1]))
+        eng.sync()
+    for k, (g, e) in enumerate(zip(grouped, alone)):
+        r = e.scan_results()
+        d = np.abs(g.scan_results() - r).max() / np.abs(r).max()
+        print(f"member {k}: max |grouped - alone| / max {d:.2e}")
+        assert np.abs(r).max() > 0 and d <= 1e-6, f"member {k}: grouped vs alone {d:.2e}"
+    sp, dx, dy, w, var = jobs[1]
+    ref = O.stem_scan(sp, dx, dy, w, var, prec=32, nthreads=3)
+    assert_dets_noise_aware(grouped[1].scan_results(), ref,
+                            lambda: O.stem_scan(sp, dx, dy, w, var, prec=64, nthreads=3), 1e-4,
+                            "grouped member 1 (2048^2)")
+    grp.close()
+    for e in grouped + alone:
+        e.close()
+
+
+def test_group_refuses_mixed_grids_and_stale_members():
     a = Engine()
     b = Engine()
     c = Engine()
@@ -102,8 +136,9 @@ def test_group_refuses_mixed_or_unsupported_grids():
         Group([a, b])
     assert ex.value.code == 4
     with pytest.raises(JMSError) as ex:
-        Group([c])
-    assert ex.value.code == 5
+        Group([a, c])
+    assert ex.value.code == 4
+    Group([c]).close()  # every supported grid groups (1024: the long-column kernel)
     with pytest.raises(JMSError) as ex:
         Group([a, a])
     assert ex.value.code == 3
