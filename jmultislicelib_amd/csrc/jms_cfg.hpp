@@ -69,10 +69,6 @@ struct CfgP {
   // 0.70-1.05 ms in any of these orders
   static constexpr int XG_X = XCT == 8 ? 2 : 0;
   static constexpr int XROT_X = XCT == 8 ? 16 : 0;
-  // A-phase twiddles of the long-column kernel read whole from a full table
-  // (16-column blocks) instead of two half-table reads and a product: 6%
-  // fewer VALU instructions, yet 0.706 -> 0.723 ms at N = 2048 (off)
-  static constexpr bool XTWF = false;
   // columns per block of the column kernel in use (partial-sum tiles)
   static constexpr int CC = XL ? XCT : (WL ? CW : C);
   static constexpr int PADSH = 3;

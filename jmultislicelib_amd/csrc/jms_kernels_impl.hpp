@@ -899,16 +899,6 @@ __device__ __forceinline__ void twiddle_a(float2 (&v)[M], const float2* tw, int 
   });
 }
 
-// the same from a per-thread row of the table W_N^(n1*k), k = 0..M-1 (one
-// read per factor at a constant offset)
-template <int M, int DIR>
-__device__ __forceinline__ void twiddle_a_row(float2 (&v)[M], const float2* row) {
-  static_for<1, M>([&](auto K) {
-    constexpr int k = decltype(K)::value;
-    v[k] = (DIR < 0) ? cmul(v[k], row[k]) : cmulconj(v[k], row[k]);
-  });
-}
-
 template <int N, bool INIT, bool EXIT, bool DET, class C_ = Cfg<N>, bool PANEL = false, bool GRP = false>
 __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4))) void k_col_pass_x(ColArgs a0) {
   constexpr int M = N / 64;
@@ -918,20 +908,13 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
   static_assert(CT == 16 || CT == 8, "long-column kernel: 8 or 16 columns per block");
   constexpr int NT = 64 * CT;  // threads
   __shared__ float2 s_x[16 * 64 * CT];  // one round: [k2 % 16][n1][column]
-  // TWF: W_N^(n1*k) at [n1][k] (rows padded to M + 1: the four values of
-  // n1 in a wavefront fall on distinct banks); else W_N^e, e < N/2
-  constexpr bool TWF = C_::XTWF && CT == 16;
-  __shared__ float2 s_tw[TWF ? 64 * (M + 1) : N / 2];
+  __shared__ float2 s_tw[N / 2];  // W_N^e, e < N/2
   // W_64^(j*c) at [c][j] (j = 0..3, c = 0..15): the four lanes of a quad
   // read four adjacent entries (in s_tw they would share one bank)
   __shared__ float2 s_w64[64];
   __shared__ float s_red[16 * kMaxDet];
   const int tid = threadIdx.x;
-  if constexpr (TWF) {
-    for (int e = tid; e < 64 * M; e += NT) s_tw[(e / M) * (M + 1) + e % M] = a0.twg[(e / M) * (e % M)];
-  } else {
-    for (int e = tid; e < N / 2; e += NT) s_tw[e] = a0.twg[e];
-  }
+  for (int e = tid; e < N / 2; e += NT) s_tw[e] = a0.twg[e];
   if (tid < 64) s_w64[tid] = a0.twg[((tid & 3) * (tid >> 2) * M) % N];
   int bx, by;
   if constexpr (C_::XG_X > 0) block_xy_groups<C_::XG_X, C_::XROT_X>(bx, by);
@@ -1012,10 +995,7 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
   }
   __builtin_amdgcn_sched_barrier(0);
   __syncthreads();  // twiddle table complete
-  if constexpr (!INIT) {
-    if constexpr (TWF) twiddle_a_row<M, -1>(v, s_tw + n1 * (M + 1));
-    else twiddle_a<M, -1>(v, s_tw, opaque(n1));
-  }
+  if constexpr (!INIT) twiddle_a<M, -1>(v, s_tw, opaque(n1));
   float acc[kMaxDet];
 #pragma unroll
   for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.f;
@@ -1158,8 +1138,7 @@ __global__ __launch_bounds__(64 * C_::XCT) __attribute__((amdgpu_waves_per_eu(4)
       });
     });
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (TWF) twiddle_a_row<M, +1>(v, s_tw + opaque(n1) * (M + 1));
-    else twiddle_a<M, +1>(v, s_tw, opaque(n1));
+    twiddle_a<M, +1>(v, s_tw, opaque(n1));
     DFT<M, +1>::run(v);
     if (ca < ncol) {
       static_for<0, M>([&](auto I) {

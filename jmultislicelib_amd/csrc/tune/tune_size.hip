@@ -55,14 +55,13 @@ struct ColV : B {
   } while (0)
 
 // long-column kernel: CT columns per block, block order (groups of G waves
-// rotated by ROT tiles; G = 0 the hardware order), full twiddle table TWF
-template <int CT, int G, int ROT, bool TWF = false>
+// rotated by ROT tiles; G = 0 the hardware order)
+template <int CT, int G, int ROT>
 struct XcolV : B {
   static constexpr int XCT = CT;
   static constexpr int CC = CT;
   static constexpr int XG_X = G;
   static constexpr int XROT_X = ROT;
-  static constexpr bool XTWF = TWF;
 };
 
 struct Bufs {
@@ -257,8 +256,8 @@ int main() {
       using CF = decltype(cf);
       float best = 1e30f;
       for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
-      std::printf("{\"n\": %d, \"pass\": \"xcol\", \"ct\": %d, \"g\": %d, \"rot\": %d, \"twf\": %d, "
-                  "\"ms\": %.5f, \"gbs\": %.1f}\n", N, CF::XCT, CF::XG_X, CF::XROT_X, (int)CF::XTWF, best,
+      std::printf("{\"n\": %d, \"pass\": \"xcol\", \"ct\": %d, \"g\": %d, \"rot\": %d, "
+                  "\"ms\": %.5f, \"gbs\": %.1f}\n", N, CF::XCT, CF::XG_X, CF::XROT_X, best,
                   bytes / (best * 1e-3) / 1e9);
     };
     xcol(XcolV<16, 0, 0>{});

@@ -18,9 +18,9 @@ def _lines(n, row_ms=1.0, flat_ms=1.0, panel=(1.0, 1.0, 1.0, 1.0), x16=None, x8=
            {"n": n, "pass": "layout", "row_ms": panel[0], "row_panel_ms": panel[1], "col_ms": panel[2],
             "col_panel_ms": panel[3]}]
     if x16 is not None:
-        out.append({"n": n, "pass": "xcol", "ct": 16, "g": 0, "rot": 0, "twf": 0, "ms": x16})
-        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "twf": 0, "ms": x8})
-        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 1, "rot": 8, "twf": 0, "ms": 0.1 * x8})  # not selectable
+        out.append({"n": n, "pass": "xcol", "ct": 16, "g": 0, "rot": 0, "ms": x16})
+        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "ms": x8})
+        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 1, "rot": 8, "ms": 0.1 * x8})  # not selectable
     return out
 
 
