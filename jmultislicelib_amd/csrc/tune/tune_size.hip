@@ -241,6 +241,14 @@ int main() {
     // no LDS padding: every exchange address a constant offset from one base
     col_variant<ColV<16, 1, 0, B::TWR, 24>>(b, 16, 1, bytes);
     if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 24>>(b, 8, 1, bytes);
+    // one padding slot per 4 / 16 / 32 elements (per 4: exchange offsets stay
+    // constants where the strides are multiples of 4)
+    col_variant<ColV<16, 1, 0, B::TWR, 2>>(b, 16, 1, bytes);
+    if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 2>>(b, 8, 1, bytes);
+    col_variant<ColV<16, 1, 0, B::TWR, 4>>(b, 16, 1, bytes);
+    if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 4>>(b, 8, 1, bytes);
+    col_variant<ColV<16, 1, 0, B::TWR, 5>>(b, 16, 1, bytes);
+    if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 5>>(b, 8, 1, bytes);
     // 128 threads per column (two wavefronts, block barriers in the
     // transform): every radix stage in one round, far fewer registers
     if constexpr (B::T > 64 && B::T <= 128) {
