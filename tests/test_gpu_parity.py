@@ -184,6 +184,41 @@ def test_detectors_without_masks_vs_oracle():
     eng.close()
 
 
+def test_offset_defocus_tilt_phase_plate_vs_oracle():
+    """OffsetIncomingWave with a defocus, a beam-tilt pixel shift and a
+    constant phase plate (bcppl on and off) on the single-wave path, and a
+    staged batch with a defocus per position and the phase plate (fused into
+    the first column pass); every case against the oracle's offset wave run
+    through its slice loop."""
+    sp = random_setup(96, 64, nslices=5, nvar=(2, 2), ndetper=2, seed=61)
+    ax, ay, wl = sp.a0[0], sp.a0[1], sp.wl
+    cppl = np.random.default_rng(9).uniform(-0.6, 0.6, (sp.ny, sp.nx)).astype(np.float32)
+    var = configs.variant_sequences(sp, 3, seed=2)
+    eng = Engine().setup(sp)
+    eng._c("jms_hip_set_phaseplate", cppl.ctypes.data)
+    for k, (dx, dy, dz, bt, use) in enumerate([(0.21, 0.13, 4.0, (2, -3), True),
+                                               (0.05, 0.4, -6.0, (0, 0), False),
+                                               (0.33, 0.02, 0.0, (-1, 1), True)]):
+        eng.wave_offset(dx, dy, dz, bt[0], bt[1], use)
+        eng.multislice(var[k], 1.0, False)
+        win = O.offset_wave(sp.probe, ax, ay, wl, dx, dy, dz, cppl=cppl if use else None, ibtx=bt[0], ibty=bt[1])
+        ref, _ = O.multislice(sp, win, var[k], 1.0)
+        assert_dets(eng.result(1), ref, TOL, f"single wave, case {k}")
+    dx = np.array([0.21, 0.05, 0.33], np.float32)
+    dy = np.array([0.13, 0.4, 0.02], np.float32)
+    dz = np.array([4.0, -6.0, 0.0], np.float32)
+    eng.set_batch(3)
+    eng.stage_scan(dx, dy, dz, np.ones(1, np.float32), var)
+    eng.run_staged(0, 3)
+    eng.sync()
+    gs = eng.scan_results()
+    for p in range(3):
+        win = O.offset_wave(sp.probe, ax, ay, wl, float(dx[p]), float(dy[p]), float(dz[p]), cppl=cppl)
+        ref, _ = O.multislice(sp, win, var[p], 1.0)
+        assert_dets(gs[p], ref, TOL, f"staged position {p}")
+    eng.close()
+
+
 def test_zero_object_slices_read_the_incident_wave():
     """No object slices (ObjectSliceSetup with an empty list, a valid set-up in
     the reference): the exit plane is the entrance plane, so each detector

@@ -125,6 +125,38 @@ def test_offset_wave_shift_theorem():
     assert np.abs(r1 - np.roll(np.roll(r0, dy_pix, 0), dx_pix, 1)).max() < 1e-6 * np.abs(r0).max()
 
 
+def test_offset_wave_defocus_phase_plate_and_tilt_closed_form():
+    """OffsetIncomingWave (JMultiSlice.cpp:1961-2084, MulPhasePlateC01Kernel
+    ArrayOps.cu:406-422): psi(k) <- psi0(k - t) exp(-i[2 pi (kx dx + ky dy)
+    + pi lambda dz k^2 + cppl(k)]), the beam-tilt pixel shift t = (ibtx, ibty)
+    cyclic, frequencies in FFT order; each term alone and all together
+    against a float64 numpy evaluation."""
+    sp = random_setup(40, 30, nslices=1, seed=4)
+    ax, ay, wl = sp.a0[0], sp.a0[1], sp.wl
+    kx = np.fft.fftfreq(sp.nx, d=ax / sp.nx)[None, :]
+    ky = np.fft.fftfreq(sp.ny, d=ay / sp.ny)[:, None]
+    k2 = kx ** 2 + ky ** 2
+    p0 = sp.probe.astype(np.complex128)
+    cppl = np.random.default_rng(3).uniform(-np.pi, np.pi, (sp.ny, sp.nx)).astype(np.float32)
+    scale = np.abs(p0).max()
+
+    def want(dx=0.0, dy=0.0, dz=0.0, cp=None, bt=(0, 0)):
+        w = np.roll(p0, (bt[1], bt[0]), axis=(0, 1))
+        chi = 2 * np.pi * (kx * dx + ky * dy) + np.pi * wl * dz * k2 + (0.0 if cp is None else cp)
+        return w * np.exp(-1j * chi)
+
+    cases = [dict(dz=3.0), dict(cp=cppl), dict(bt=(2, -3)), dict(dx=0.07, dy=-0.11, dz=-2.5, cp=cppl, bt=(-1, 4))]
+    for c in cases:
+        bt = c.get("bt", (0, 0))
+        got = O.offset_wave(sp.probe, ax, ay, wl, c.get("dx", 0.0), c.get("dy", 0.0), c.get("dz", 0.0),
+                            cppl=c.get("cp"), ibtx=bt[0], ibty=bt[1])
+        err = np.abs(got - want(**c)).max() / scale
+        assert err < 2e-5, (sorted(c), err)
+    # a wrong sign of the defocus term is caught
+    bad = np.abs(O.offset_wave(sp.probe, ax, ay, wl, 0.0, 0.0, 3.0) - want(dz=-3.0)).max() / scale
+    assert bad > 1e-2
+
+
 def numpy_channels(sp, wave_in, var, weight):
     """float64 numpy evaluation of the intensity and wave channels
     (ReadoutDifDet_h / ReadoutImgDet_h, JMultiSlice.cpp:3369-3538): Fourier-space
