@@ -89,7 +89,9 @@ int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro);
 /* object slice sequence (structure slice index per object slice) -- :522 */
 int jms_hip_setup_objects(jms_hip_engine* e, int nobjslc, const int* objslc);
 /* detection: det_objslc[nobjslc+1] maps planes to detection indices
- * (-1 = none), ndet integrating detectors, flags = JMS_HIP_DETECT_* -- :668 */
+ * (-1 = none), ndet integrating detectors, flags = JMS_HIP_DETECT_* -- :668.
+ * Codes: 5 an index >= ndetslc, 6 an index used by two planes (the
+ * reference would add both planes' readouts into it). */
 int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objslc, int ndet,
                             int flags);
 /* detector function det[nx*ny]; with msklen>0 only the listed pixel indices

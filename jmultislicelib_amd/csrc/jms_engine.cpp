@@ -672,8 +672,14 @@ int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objsl
   if (!det_objslc || ndetslc < 1 || ndet < 0) return e->fail(2, "invalid detection setup");
   if (ndet > jms::kMaxDetPerSweep) return e->fail(3, "too many integrating detectors (max 8)");
   if (flags & ~JMS_HIP_DETECT_ALL) return e->fail(4, "unknown detection channel flag");
-  for (int i = 0; i <= e->nobjslc; ++i)
-    if (det_objslc[i] >= ndetslc) return e->fail(5, "detection index out of range");
+  std::vector<char> used(ndetslc, 0);
+  for (int i = 0; i <= e->nobjslc; ++i) {
+    const int id = det_objslc[i];
+    if (id >= ndetslc) return e->fail(5, "detection index out of range");
+    // every plane's readout fills its index's partial sums: an index shared
+    // by two planes would keep only the later one
+    if (id >= 0 && used[id]++) return e->fail(6, "a detection index used by two planes");
+  }
   (void)hipSetDevice(e->device);
   e->drop_graphs();
   e->ndetslc = ndetslc;

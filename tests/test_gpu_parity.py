@@ -219,6 +219,18 @@ def test_offset_defocus_tilt_phase_plate_vs_oracle():
     eng.close()
 
 
+def test_detection_setup_refuses_shared_or_out_of_range_indices():
+    sp = random_setup(64, 64, nslices=4, nvar=(2, 1), ndetper=2, seed=3)
+    eng = Engine().setup(sp)
+    for plane_ids, ndetslc, code in (([0, -1, 0, -1, 1], 3, 6), ([0, -1, 1, -1, 3], 3, 5)):
+        ids = np.array(plane_ids, np.int32)
+        rc = eng.L.jms_hip_setup_detection(eng.h, ndetslc, ids.ctypes.data, 3, 1)
+        assert rc == code, (plane_ids, rc)
+    ids = np.array([0, -1, 1, -1, 2], np.int32)
+    assert eng.L.jms_hip_setup_detection(eng.h, 3, ids.ctypes.data, 3, 1) == 0
+    eng.close()
+
+
 def test_zero_object_slices_read_the_incident_wave():
     """No object slices (ObjectSliceSetup with an empty list, a valid set-up in
     the reference): the exit plane is the entrance plane, so each detector
