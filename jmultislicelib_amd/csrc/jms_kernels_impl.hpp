@@ -848,7 +848,8 @@ __global__ __launch_bounds__(C_::TW * C_::CW) __attribute__((amdgpu_waves_per_eu
 // The phase plate (INIT), propagator, readouts and the exit store act in the
 // B layout on row-major tables (a wavefront touches four 128-byte or eight
 // 64-byte row segments per access); the inverse transform runs the steps
-// backwards.
+// backwards.  GRP: block row by is wave w0 + by of a grouped launch's wave
+// map, whose job and plane tables supply the buffers and tables.
 // grid: (ceil(nx/CT), nwave); block: 64*CT
 // ---------------------------------------------------------------------------
 // the value of x, hidden from the optimiser: table reads indexed with it are
