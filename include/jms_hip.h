@@ -10,7 +10,9 @@
  *
  * An engine owns one HIP stream on the device that is current when it is
  * created.  Calls on one engine must not overlap; distinct engines are
- * independent.
+ * independent.  Batch runs are enqueued on that stream and return at once;
+ * the set-up and upload calls first wait for the work already on it, so a
+ * new scan, probe or table never replaces data a queued launch still reads.
  */
 #ifndef JMS_HIP_H
 #define JMS_HIP_H

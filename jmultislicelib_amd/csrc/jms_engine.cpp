@@ -435,6 +435,7 @@ int jms_hip_set_grid(jms_hip_engine* e, int nx, int ny, float a0x, float a0y, fl
   if (!jms_hip_grid_supported(nx, ny)) return e->fail(2, "grid size has no compiled kernels");
   if (!(a0x > 0.f) || !(a0y > 0.f)) return e->fail(3, "invalid super-cell size");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   e->drop_graphs();
   e->nx = nx; e->ny = ny; e->n = (long long)nx * ny;
   e->a0x = a0x; e->a0y = a0y; e->wl = wl;
@@ -457,6 +458,7 @@ int jms_hip_setup_gratings(jms_hip_engine* e, int npgx, int npgy, int nslc, cons
   if (npgx != e->nx || npgy != e->ny)
     return e->fail(3, "phase-grating grid differs from the wave grid (sub-frame gratings are not supported)");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   e->drop_graphs();
   e->npgx = npgx; e->npgy = npgy; e->nscslc = nslc;
   e->nvarslc.assign(nvarslc, nvarslc + nslc);
@@ -480,6 +482,7 @@ int jms_hip_upload_gratings(jms_hip_engine* e, int islc, int nvar, const float* 
   const int nl = std::min(nvar, e->nvarslc[islc]);
   const size_t nb = sizeof(float) * 2 * (size_t)nl * e->npgx * e->npgy;
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   hipError_t err = hipMemcpy(e->pgr.as<float2>() + e->slcoffset[islc], pgr, nb, hipMemcpyHostToDevice);
   if (err != hipSuccess) return e->fail(102, "upload phase gratings", err);
   return 0;
@@ -539,6 +542,7 @@ int jms_hip_gaussian_gratings(jms_hip_engine* e, int islc, int nvar, const int* 
   if (idx.empty()) idx.push_back(0);
   if (atoms.empty()) atoms.push_back(jms::GaussAtom{});
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   DevBuf d_atoms, d_off, d_idx;
   hipError_t err = d_atoms.ensure(sizeof(jms::GaussAtom) * atoms.size());
   if (err == hipSuccess) err = d_off.ensure(sizeof(int) * off.size());
@@ -596,6 +600,7 @@ int jms_hip_formfactor_gratings(jms_hip_engine* e, int islc, int nvar, int ntype
   const float sc = POT_SCA / vol, sg = PGR_SIG * cz * e->wl;
   const float ftrn = 1.f / (float)nx / (float)ny;
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   DevBuf d_ff, d_atoms, d_off;
   hipError_t err = d_ff.ensure(sizeof(float) * 2 * n * ntype);
   if (err == hipSuccess) err = d_atoms.ensure(sizeof(jms::FFAtom) * atoms.size());
@@ -632,6 +637,7 @@ int jms_hip_setup_propagators(jms_hip_engine* e, int npro, int nslc, const int* 
   for (int i = 0; i < nslc; ++i)
     if (proidx[i] < 0 || proidx[i] >= npro) return e->fail(3, "propagator index out of range");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   e->drop_graphs();
   e->npro = npro;
   e->proidx.assign(proidx, proidx + nslc);
@@ -645,6 +651,7 @@ int jms_hip_upload_propagator(jms_hip_engine* e, int ipro, const float* pro) {
   if (!e) return 1;
   if (ipro < 0 || ipro >= e->npro || !pro) return e->fail(2, "invalid propagator upload");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   hipError_t err = hipMemcpy(e->pro.as<float2>() + (long long)ipro * e->n, pro,
                              sizeof(float) * 2 * (size_t)e->n, hipMemcpyHostToDevice);
   if (err == hipSuccess) {
@@ -681,6 +688,7 @@ int jms_hip_setup_detection(jms_hip_engine* e, int ndetslc, const int* det_objsl
     if (id >= 0 && used[id]++) return e->fail(6, "a detection index used by two planes");
   }
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   e->drop_graphs();
   e->ndetslc = ndetslc;
   e->ndet = ndet;
@@ -717,6 +725,7 @@ int jms_hip_upload_detector(jms_hip_engine* e, int idet, const float* det, int m
   if (!e) return 1;
   if (idet < 0 || idet >= e->ndet || !det) return e->fail(2, "invalid detector upload");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   // effective weights: det at every listed mask pixel (a pixel listed k times
   // counts k times, as the masked sum does), or det everywhere without a mask
   std::vector<float> wgt((size_t)e->n, 0.f);
@@ -741,6 +750,7 @@ int jms_hip_set_probe(jms_hip_engine* e, const float* wav0) {
   if (!e) return 1;
   if (!wav0 || e->n <= 0) return e->fail(2, "invalid probe");
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   const size_t nb = sizeof(float) * 2 * (size_t)e->n;
   hipError_t err = e->wav0.ensure(nb);
   if (err == hipSuccess) err = hipMemcpy(e->wav0.p, wav0, nb, hipMemcpyHostToDevice);
@@ -781,6 +791,7 @@ int jms_hip_probe(jms_hip_engine* e, const float* prm, int nabrr, const float* a
   a.nab = std::min(nabrr, 24);
   for (int k = 0; k < 2 * a.nab; ++k) a.coeff[k] = abrr[k];
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   const size_t nb = sizeof(float) * 2 * (size_t)e->n;
   DevBuf partial;
   hipError_t err = e->wav0.ensure(nb);
@@ -796,6 +807,7 @@ int jms_hip_probe(jms_hip_engine* e, const float* prm, int nabrr, const float* a
 int jms_hip_set_phaseplate(jms_hip_engine* e, const float* cppl) {
   if (!e) return 1;
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   e->drop_graphs();
   if (!cppl) {
     e->have_cppl = false;
@@ -856,6 +868,7 @@ int jms_hip_stage_scan(jms_hip_engine* e, int npos, const float* dx, const float
     if (var[i] < 0 || var[i] >= e->nvarslc[js]) return e->fail(4, "variant index out of range");
   }
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   std::vector<float> pp(4 * (size_t)npos * npass);
   for (int p = 0; p < npos; ++p)
     for (int r = 0; r < npass; ++r) {
@@ -1318,6 +1331,7 @@ int jms_hip_wave_offset(jms_hip_engine* e, float dx, float dy, float dz, int ibt
 int jms_hip_wave_set(jms_hip_engine* e, const float* wav) {
   if (!e || !wav) return 1;
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   hipError_t err = e->wav.ensure(sizeof(float) * 2 * (size_t)e->n);
   if (err == hipSuccess) err = hipMemcpy(e->wav.p, wav, sizeof(float) * 2 * (size_t)e->n, hipMemcpyHostToDevice);
   if (err != hipSuccess) return e->fail(102, "set wave", err);
@@ -1468,6 +1482,7 @@ int jms_hip_get_averaging_weights(jms_hip_engine* e, float* img_weight, float* w
 int jms_hip_reset_averaging(jms_hip_engine* e, int group) {
   if (!e) return 1;
   (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->st);  // launches already enqueued may still read what this replaces
   hipError_t err = hipStreamSynchronize(e->st);
   const size_t nchan = (size_t)e->ndetslc * e->n;
   if (group == 0) {  // ResetImageAveraging (JMultiSlice.cpp:2592-2627)
