@@ -92,7 +92,11 @@ struct CfgP {
   // waves then leave the reused phase gratings and propagators in L2
   // (row pass 0.84 -> 0.78 ms per 1024 waves at N = 512)
   static constexpr bool NT_ROW = N >= 256;
-  static constexpr bool NT_COL = N >= 256;
+  // except the long-column kernel with 8-column tiles: the neighbouring
+  // tile reads the other half of each 128-byte line a little later, and a
+  // nontemporal line is gone by then (2048 columns: 0.746 -> 0.636 ms per
+  // GiB of waves on MI355X; 1024: 0.597 -> 0.582)
+  static constexpr bool NT_COL = N >= 256 && !(XL && XCT == 8);
   // LDS position policy of line layouts (0 padded, 1 swizzled) and twiddle
   // powers by product tree (one table read per butterfly) instead of one
   // table read per leg; measured at N = 512 (2048 waves, interleaved runs):

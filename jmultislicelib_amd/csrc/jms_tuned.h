@@ -247,7 +247,7 @@ template <> struct Tuned<1024> {
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
   static constexpr bool GTW = true;
-  static constexpr int CW = 16;
+  static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;

@@ -147,7 +147,7 @@ def choose(results):
         # long columns: 16 columns per block in the hardware order, or 8 in
         # the block order jms_cfg.hpp gives them (pairs of waves, 16-tile rotation)
         xcols = {r["ct"]: r for r in rs if r["pass"] == "xcol" and (r["ct"], r["g"], r["rot"]) in
-                 ((16, 0, 0), (8, 2, 16))}
+                 ((16, 0, 0), (8, 2, 16)) and r.get("nt", 1) == 1}
         if 16 in xcols and 8 in xcols and xcols[8]["ms"] < 0.98 * xcols[16]["ms"]:
             k["cw"] = 8
         plans = [r for r in rs if r["pass"] == "plan"]

@@ -56,12 +56,13 @@ struct ColV : B {
 
 // long-column kernel: CT columns per block, block order (groups of G waves
 // rotated by ROT tiles; G = 0 the hardware order)
-template <int CT, int G, int ROT>
+template <int CT, int G, int ROT, bool NT = B::NT_COL>
 struct XcolV : B {
   static constexpr int XCT = CT;
   static constexpr int CC = CT;
   static constexpr int XG_X = G;
   static constexpr int XROT_X = ROT;
+  static constexpr bool NT_COL = NT;
 };
 
 struct Bufs {
@@ -264,11 +265,21 @@ int main() {
       using CF = decltype(cf);
       float best = 1e30f;
       for (int r = 0; r < 3; ++r) best = std::fmin(best, time_col<CF>(b, 10));
-      std::printf("{\"n\": %d, \"pass\": \"xcol\", \"ct\": %d, \"g\": %d, \"rot\": %d, "
-                  "\"ms\": %.5f, \"gbs\": %.1f}\n", N, CF::XCT, CF::XG_X, CF::XROT_X, best,
+      std::printf("{\"n\": %d, \"pass\": \"xcol\", \"ct\": %d, \"g\": %d, \"rot\": %d, \"nt\": %d, "
+                  "\"ms\": %.5f, \"gbs\": %.1f}\n", N, CF::XCT, CF::XG_X, CF::XROT_X, (int)CF::NT_COL, best,
                   bytes / (best * 1e-3) / 1e9);
     };
     xcol(XcolV<16, 0, 0>{});
+    xcol(XcolV<16, 0, 0, !B::NT_COL>{});  // the other load/store cache policy
+    xcol(XcolV<8, 2, 16, !B::NT_COL>{});
+    xcol(XcolV<8, 0, 0, !B::NT_COL>{});
+    xcol(XcolV<8, 1, 32, !B::NT_COL>{});
+    xcol(XcolV<8, 1, 8, !B::NT_COL>{});
+    xcol(XcolV<8, 2, 32, !B::NT_COL>{});
+    xcol(XcolV<8, 4, 16, !B::NT_COL>{});
+    xcol(XcolV<8, 8, 16, !B::NT_COL>{});
+    xcol(XcolV<16, 1, 16, !B::NT_COL>{});
+    xcol(XcolV<16, 2, 16, !B::NT_COL>{});
     xcol(XcolV<16, 2, 16>{});
     xcol(XcolV<16, 2, 8>{});
     xcol(XcolV<16, 2, 32>{});
