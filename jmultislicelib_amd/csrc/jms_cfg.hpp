@@ -96,7 +96,9 @@ struct CfgP {
   // tile reads the other half of each 128-byte line a little later, and a
   // nontemporal line is gone by then (2048 columns: 0.746 -> 0.636 ms per
   // GiB of waves on MI355X; 1024: 0.597 -> 0.582)
-  static constexpr bool NT_COL = N >= 256 && !(XL && XCT == 8);
+  // and where the tuner found ordinary ones faster (NTC, jms_tuned.h: 480
+  // columns in 8-column half-panels 0.543 -> 0.482 ms per GiB of waves)
+  static constexpr bool NT_COL = N >= 256 && Tuned<N>::NTC && !(XL && XCT == 8);
   // LDS position policy of line layouts (0 padded, 1 swizzled) and twiddle
   // powers by product tree (one table read per butterfly) instead of one
   // table read per leg; measured at N = 512 (2048 waves, interleaved runs):

@@ -7,7 +7,8 @@
 // allocator for that kernel), TW (threads per column of that kernel; 0 =
 // default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20
 // none), PADR (the same for the row pass), PANEL (waves of staged batches in
-// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly);
+// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly),
+// NTC (nontemporal wave loads and stores in the column pass);
 // at N = 1024, 2048 CW is the long-column kernel's columns per block (16 or
 // 8).  Included by jms_cfg.hpp after the plans.
 #pragma once
@@ -37,6 +38,7 @@ struct Tuned {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 
 template <> struct TunedColPlan<360> { using type = Radices<9, 5, 8>; };
@@ -56,6 +58,7 @@ template <> struct Tuned<16> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<48> {
   static constexpr int LPB = 64;
@@ -67,6 +70,7 @@ template <> struct Tuned<48> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<64> {
   static constexpr int LPB = 64;
@@ -78,6 +82,7 @@ template <> struct Tuned<64> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<96> {
   static constexpr int LPB = 42;
@@ -89,6 +94,7 @@ template <> struct Tuned<96> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<128> {
   static constexpr int LPB = 16;
@@ -100,6 +106,7 @@ template <> struct Tuned<128> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<160> {
   static constexpr int LPB = 16;
@@ -111,6 +118,7 @@ template <> struct Tuned<160> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<192> {
   static constexpr int LPB = 8;
@@ -122,6 +130,7 @@ template <> struct Tuned<192> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<256> {
   static constexpr int LPB = 8;
@@ -133,6 +142,7 @@ template <> struct Tuned<256> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<320> {
   static constexpr int LPB = 8;
@@ -144,6 +154,7 @@ template <> struct Tuned<320> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
@@ -155,6 +166,7 @@ template <> struct Tuned<360> {
   static constexpr int PADR = 24;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = false;
 };
 template <> struct Tuned<384> {
   static constexpr int LPB = 8;
@@ -166,6 +178,7 @@ template <> struct Tuned<384> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = true;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<480> {
   static constexpr int LPB = 4;
@@ -177,6 +190,7 @@ template <> struct Tuned<480> {
   static constexpr int PADR = 24;
   static constexpr bool PANEL = true;
   static constexpr int TR = 64;
+  static constexpr bool NTC = false;
 };
 template <> struct Tuned<512> {
   static constexpr int LPB = 4;
@@ -188,6 +202,7 @@ template <> struct Tuned<512> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = true;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<625> {
   static constexpr int LPB = 4;
@@ -199,6 +214,7 @@ template <> struct Tuned<625> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -210,6 +226,7 @@ template <> struct Tuned<640> {
   static constexpr int PADR = 24;
   static constexpr bool PANEL = true;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<720> {
   static constexpr int LPB = 2;
@@ -221,6 +238,7 @@ template <> struct Tuned<720> {
   static constexpr int PADR = 24;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
@@ -232,6 +250,7 @@ template <> struct Tuned<768> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<1024> {
   static constexpr int LPB = 2;
@@ -243,6 +262,7 @@ template <> struct Tuned<1024> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 template <> struct Tuned<2048> {
   static constexpr int LPB = 1;
@@ -254,6 +274,7 @@ template <> struct Tuned<2048> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
+  static constexpr bool NTC = true;
 };
 
 }  // namespace jms

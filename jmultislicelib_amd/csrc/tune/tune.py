@@ -150,6 +150,18 @@ def choose(results):
                  ((16, 0, 0), (8, 2, 16)) and r.get("nt", 1) == 1}
         if 16 in xcols and 8 in xcols and xcols[8]["ms"] < 0.98 * xcols[16]["ms"]:
             k["cw"] = 8
+        # the column pass's loads and stores without the nontemporal hint,
+        # timed in the layout the staged batches use
+        cnt = next((r for r in rs if r["pass"] == "col_nt" and r["nt"] == 0), None)
+        if cnt and lay:
+            cur = lay["col_panel_ms"] if k.get("panel") else lay["col_ms"]
+            alt = cnt["panel_ms"] if k.get("panel") else cnt["ms"]
+            if alt < 0.98 * cur:
+                k["ntc"] = False
+        elif cnt:
+            cur = next((r["ms"] for r in rs if r["pass"] == "col"), None)
+            if cur and cnt["ms"] < 0.98 * cur:
+                k["ntc"] = False
         plans = [r for r in rs if r["pass"] == "plan"]
         if plans:  # the row and the column transforms each take their fastest plan
             dname = ",".join(map(str, default_plan(n)))
@@ -175,7 +187,8 @@ def emit(results, path):
              "// allocator for that kernel), TW (threads per column of that kernel; 0 =",
              "// default), PADC (LDS padding shift of that kernel's tile; 0 = default, >= 20",
              "// none), PADR (the same for the row pass), PANEL (waves of staged batches in",
-             "// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly);",
+             "// 16-column panels), TR (row-pass threads per line; 0 = one per butterfly),",
+             "// NTC (nontemporal wave loads and stores in the column pass);",
              "// at N = 1024, 2048 CW is the long-column kernel's columns per block (16 or",
              "// 8).  Included by jms_cfg.hpp after the plans.",
              "#pragma once", "", "namespace jms {", "",
@@ -200,6 +213,7 @@ def emit(results, path):
              "  static constexpr int PADR = 0;",
              "  static constexpr bool PANEL = false;",
              "  static constexpr int TR = 0;",
+             "  static constexpr bool NTC = true;",
              "};", ""]
     for n, k in ch.items():
         if "col_plan" in k:
@@ -208,7 +222,7 @@ def emit(results, path):
         if "plan" in k:
             lines.append(f"template <> struct TunedPlan<{n}> {{ using type = Radices<{k['plan'].replace(',', ', ')}>; }};")
     for n, k in ch.items():
-        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad", "padr", "panel", "tr"} & set(k):
+        if not {"lpb", "gtw", "cw", "wpe", "tw", "pad", "padr", "panel", "tr", "ntc"} & set(k):
             continue
         lines.append(f"template <> struct Tuned<{n}> {{")
         lines.append(f"  static constexpr int LPB = {k.get('lpb', 0)};")
@@ -220,6 +234,7 @@ def emit(results, path):
         lines.append(f"  static constexpr int PADR = {k.get('padr', 0)};")
         lines.append(f"  static constexpr bool PANEL = {'true' if k.get('panel') else 'false'};")
         lines.append(f"  static constexpr int TR = {k.get('tr', 0)};")
+        lines.append(f"  static constexpr bool NTC = {'false' if k.get('ntc') is False else 'true'};")
         lines.append("};")
     lines += ["", "}  // namespace jms", ""]
     open(path, "w").write("\n".join(lines))

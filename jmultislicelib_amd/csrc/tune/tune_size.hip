@@ -35,8 +35,9 @@ struct RowP : B {
 };
 // column variants (wave-local kernel): columns per block x waves per SIMD x
 // threads per column (0: default)
-template <int CWV, int WPE, int TWV = 0, bool TWRV = B::TWR, int PADV = B::PADSH_COL>
+template <int CWV, int WPE, int TWV = 0, bool TWRV = B::TWR, int PADV = B::PADSH_COL, bool NTV = B::NT_COL>
 struct ColV : B {
+  static constexpr bool NT_COL = NTV;
   static constexpr int CW = CWV;
   static constexpr int CC = CWV;
   static constexpr int WPE_COL = WPE;
@@ -242,6 +243,17 @@ int main() {
     // no LDS padding: every exchange address a constant offset from one base
     col_variant<ColV<16, 1, 0, B::TWR, 24>>(b, 16, 1, bytes);
     if constexpr (B::TW % 8 == 0) col_variant<ColV<8, 1, 0, B::TWR, 24>>(b, 8, 1, bytes);
+    // the tuned shape with the other load/store cache policy, in both layouts
+    {
+      using CN = ColV<B::CW, B::WPE_COL, B::TW, B::TWR, B::PADSH_COL, !B::NT_COL>;
+      float bn = 1e30f, bp = 1e30f;
+      for (int r = 0; r < 3; ++r) {
+        bn = std::fmin(bn, time_col<CN, false>(b, 10));
+        if constexpr (N % 16 == 0) bp = std::fmin(bp, time_col<CN, true>(b, 10));
+      }
+      std::printf("{\"n\": %d, \"pass\": \"col_nt\", \"cw\": %d, \"nt\": %d, \"ms\": %.5f, \"panel_ms\": %.5f}\n",
+                  N, B::CW, (int)CN::NT_COL, bn, bp);
+    }
     // one padding slot per 4 / 16 / 32 elements (per 4: exchange offsets stay
     // constants where the strides are multiples of 4)
     col_variant<ColV<16, 1, 0, B::TWR, 2>>(b, 16, 1, bytes);

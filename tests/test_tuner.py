@@ -16,7 +16,8 @@ def _lines(n, row_ms=1.0, flat_ms=1.0, panel=(1.0, 1.0, 1.0, 1.0), x16=None, x8=
            {"n": n, "pass": "row", "variant": "tuned", "lpb": 4, "gtw": 0, "ms": 1.0},
            {"n": n, "pass": "row", "variant": "tuned_flat", "lpb": 4, "gtw": 0, "ms": flat_ms},
            {"n": n, "pass": "layout", "row_ms": panel[0], "row_panel_ms": panel[1], "col_ms": panel[2],
-            "col_panel_ms": panel[3]}]
+            "col_panel_ms": panel[3]},
+           {"n": n, "pass": "col_nt", "cw": 8, "nt": 0, "ms": 1.0, "panel_ms": 0.8 if n == 480 else 1.0}]
     if x16 is not None:
         out.append({"n": n, "pass": "xcol", "ct": 16, "g": 0, "rot": 0, "ms": x16})
         out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "ms": x8})
@@ -28,7 +29,7 @@ def test_choices_per_length():
     res = (_lines(480, flat_ms=0.9, panel=(1.0, 0.95, 1.0, 0.9)) + _lines(512, flat_ms=0.99) +
            _lines(1024, x16=0.706, x8=0.597) + _lines(2048, x16=0.706, x8=0.70))
     ch = T.choose(res)
-    assert ch[480] == {"padr": 24, "panel": True}
+    assert ch[480] == {"padr": 24, "panel": True, "ntc": False}
     assert ch[512] == {}  # 1% faster: kept out
     assert ch[1024] == {"cw": 8}
     assert ch[2048] == {}
@@ -41,6 +42,6 @@ def test_emit_writes_chosen_fields(tmp_path):
     txt = path.read_text()
     i = txt.index("template <> struct Tuned<480>")
     blk = txt[i:txt.index("};", i)]
-    assert "PADR = 24;" in blk and "PANEL = true;" in blk
+    assert "PADR = 24;" in blk and "PANEL = true;" in blk and "NTC = false;" in blk
     i = txt.index("template <> struct Tuned<1024>")
     assert "CW = 8;" in txt[i:txt.index("};", i)]
