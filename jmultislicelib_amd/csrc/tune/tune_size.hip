@@ -21,6 +21,10 @@ struct RowV : B {
   static constexpr bool GTW_ROW = G;
   static constexpr bool TWR_ROW = G;
 };
+// the tuned row shape with the other load/store cache policy
+struct RowNT : B {
+  static constexpr bool NT_ROW = !B::NT_ROW;
+};
 // rows on TRV threads (several butterflies per thread where a stage has more)
 template <int TRV, int LPBV>
 struct RowT : B {
@@ -220,6 +224,15 @@ int main() {
   if (LH != L0) {
     row_variant<RowV<LH, false>>(b, "half", LH, false, bytes);
     row_variant<RowV<LH, true>>(b, "half", LH, true, bytes);
+  }
+  {  // the row pass with the other cache policy, in both layouts
+    float bn = 1e30f, bp = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      bn = std::fmin(bn, time_row<RowNT, false>(b, 10));
+      if constexpr (N % 16 == 0) bp = std::fmin(bp, time_row<RowNT, true>(b, 10));
+    }
+    std::printf("{\"n\": %d, \"pass\": \"row_nt\", \"nt\": %d, \"ms\": %.5f, \"panel_ms\": %.5f}\n", N,
+                (int)RowNT::NT_ROW, bn, bp);
   }
   if constexpr (B::WL && !B::XL) {
     col_variant<ColV<16, 1>>(b, 16, 1, bytes);
