@@ -63,10 +63,12 @@ struct CfgP {
   static constexpr int XCT = (XL && CW == 8) ? 8 : 16;
   // block order of the long-column kernel: XG_X > 0 takes block_xy_groups
   // with groups of XG_X waves rotated by XROT_X tiles, 0 the hardware order.
-  // Per 1 GiB of waves on MI355X (tuner): N = 1024 16 columns 0.706 ms, 8
-  // columns in the hardware order 0.793 ms, in groups of 1 / 2 / 4 waves
-  // 0.707 / 0.597 / 0.618 ms; N = 2048 16 columns 0.706 ms, 8 columns
-  // 0.70-1.05 ms in any of these orders
+  // Per 1 GiB of waves on MI355X (tuner, nontemporal loads and stores):
+  // N = 1024 16 columns 0.706 ms, 8 columns in the hardware order 0.793 ms,
+  // in groups of 1 / 2 / 4 waves 0.707 / 0.597 / 0.618 ms; N = 2048 16
+  // columns 0.706 ms, 8 columns 0.70-1.05 ms.  With ordinary loads and
+  // stores (NT_COL below) 8 columns in groups of 2 waves take 0.582 ms at
+  // 1024 and 0.636 ms at 2048, which both lengths use
   static constexpr int XG_X = XCT == 8 ? 2 : 0;
   static constexpr int XROT_X = XCT == 8 ? 16 : 0;
   // columns per block of the column kernel in use (partial-sum tiles)
