@@ -144,10 +144,12 @@ def choose(results):
             b = min(cols, key=lambda r: r["ms"])
             if b["ms"] < 0.98 * d["ms"]:
                 k["cw"], k["wpe"], k["tw"], k["pad"] = b["cw"], b["wpe"], b.get("tw", 0), b.get("pad", 3)
-        # long columns: 16 columns per block in the hardware order, or 8 in
-        # the block order jms_cfg.hpp gives them (pairs of waves, 16-tile rotation)
-        xcols = {r["ct"]: r for r in rs if r["pass"] == "xcol" and (r["ct"], r["g"], r["rot"]) in
-                 ((16, 0, 0), (8, 2, 16)) and r.get("nt", 1) == 1}
+        # long columns: 16 columns per block in the hardware order with
+        # nontemporal loads and stores, or 8 in the block order and cache
+        # policy jms_cfg.hpp gives them (pairs of waves, 16-tile rotation,
+        # ordinary loads and stores)
+        xcols = {r["ct"]: r for r in rs if r["pass"] == "xcol" and (r["ct"], r["g"], r["rot"], r.get("nt", 1)) in
+                 ((16, 0, 0, 1), (8, 2, 16, 0))}
         if 16 in xcols and 8 in xcols and xcols[8]["ms"] < 0.98 * xcols[16]["ms"]:
             k["cw"] = 8
         # the column pass's loads and stores without the nontemporal hint,

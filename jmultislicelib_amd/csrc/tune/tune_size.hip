@@ -61,7 +61,8 @@ struct ColV : B {
 
 // long-column kernel: CT columns per block, block order (groups of G waves
 // rotated by ROT tiles; G = 0 the hardware order)
-template <int CT, int G, int ROT, bool NT = B::NT_COL>
+// (default cache policy: the one jms_cfg.hpp gives a tile of CT columns)
+template <int CT, int G, int ROT, bool NT = (CT == 16 && N >= 256 && Tuned<N>::NTC)>
 struct XcolV : B {
   static constexpr int XCT = CT;
   static constexpr int CC = CT;
@@ -295,16 +296,11 @@ int main() {
                   bytes / (best * 1e-3) / 1e9);
     };
     xcol(XcolV<16, 0, 0>{});
-    xcol(XcolV<16, 0, 0, !B::NT_COL>{});  // the other load/store cache policy
-    xcol(XcolV<8, 2, 16, !B::NT_COL>{});
-    xcol(XcolV<8, 0, 0, !B::NT_COL>{});
-    xcol(XcolV<8, 1, 32, !B::NT_COL>{});
-    xcol(XcolV<8, 1, 8, !B::NT_COL>{});
-    xcol(XcolV<8, 2, 32, !B::NT_COL>{});
-    xcol(XcolV<8, 4, 16, !B::NT_COL>{});
-    xcol(XcolV<8, 8, 16, !B::NT_COL>{});
-    xcol(XcolV<16, 1, 16, !B::NT_COL>{});
-    xcol(XcolV<16, 2, 16, !B::NT_COL>{});
+    // the other load/store cache policy for each tile width
+    xcol(XcolV<16, 0, 0, false>{});
+    xcol(XcolV<16, 2, 16, false>{});
+    xcol(XcolV<8, 2, 16, true>{});
+    xcol(XcolV<8, 1, 32, true>{});
     xcol(XcolV<16, 2, 16>{});
     xcol(XcolV<16, 2, 8>{});
     xcol(XcolV<16, 2, 32>{});

@@ -19,9 +19,10 @@ def _lines(n, row_ms=1.0, flat_ms=1.0, panel=(1.0, 1.0, 1.0, 1.0), x16=None, x8=
             "col_panel_ms": panel[3]},
            {"n": n, "pass": "col_nt", "cw": 8, "nt": 0, "ms": 1.0, "panel_ms": 0.8 if n == 480 else 1.0}]
     if x16 is not None:
-        out.append({"n": n, "pass": "xcol", "ct": 16, "g": 0, "rot": 0, "ms": x16})
-        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "ms": x8})
-        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 1, "rot": 8, "ms": 0.1 * x8})  # not selectable
+        out.append({"n": n, "pass": "xcol", "ct": 16, "g": 0, "rot": 0, "nt": 1, "ms": x16})
+        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "nt": 0, "ms": x8})
+        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 1, "rot": 8, "nt": 0, "ms": 0.1 * x8})  # not selectable
+        out.append({"n": n, "pass": "xcol", "ct": 8, "g": 2, "rot": 16, "nt": 1, "ms": 0.1 * x8})  # other policy
     return out
 
 
