@@ -159,7 +159,7 @@ template <> struct Tuned<320> {
 template <> struct Tuned<360> {
   static constexpr int LPB = 0;
   static constexpr bool GTW = false;
-  static constexpr int CW = 16;
+  static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 24;
@@ -214,7 +214,7 @@ template <> struct Tuned<625> {
   static constexpr int PADR = 0;
   static constexpr bool PANEL = false;
   static constexpr int TR = 0;
-  static constexpr bool NTC = true;
+  static constexpr bool NTC = false;
 };
 template <> struct Tuned<640> {
   static constexpr int LPB = 3;
@@ -243,12 +243,12 @@ template <> struct Tuned<720> {
 template <> struct Tuned<768> {
   static constexpr int LPB = 2;
   static constexpr bool GTW = true;
-  static constexpr int CW = 16;
+  static constexpr int CW = 8;
   static constexpr int WPE = 1;
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
-  static constexpr bool PANEL = false;
+  static constexpr bool PANEL = true;
   static constexpr int TR = 0;
   static constexpr bool NTC = true;
 };

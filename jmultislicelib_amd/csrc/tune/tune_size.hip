@@ -268,6 +268,29 @@ int main() {
       std::printf("{\"n\": %d, \"pass\": \"col_nt\", \"cw\": %d, \"nt\": %d, \"ms\": %.5f, \"panel_ms\": %.5f}\n",
                   N, B::CW, (int)CN::NT_COL, bn, bp);
     }
+    // 16 and 8 columns per block, each with both cache policies, in both layouts
+    {
+      auto cn = [&](auto cf) {
+        using CF = decltype(cf);
+        float bn = 1e30f, bp = 1e30f;
+        if constexpr ((CF::TW * CF::CW) % 64 == 0) {
+          for (int r = 0; r < 3; ++r) {
+            bn = std::fmin(bn, time_col<CF, false>(b, 10));
+            if constexpr (N % 16 == 0) bp = std::fmin(bp, time_col<CF, true>(b, 10));
+          }
+        }
+        std::printf("{\"n\": %d, \"pass\": \"col_cw_nt\", \"cw\": %d, \"nt\": %d, \"ms\": %.5f, "
+                    "\"panel_ms\": %.5f}\n", N, CF::CW, (int)CF::NT_COL, bn, bp);
+      };
+      if constexpr ((B::TW * 16) % 64 == 0) {
+        cn(ColV<16, B::WPE_COL, B::TW, B::TWR, B::PADSH_COL, true>{});
+        cn(ColV<16, B::WPE_COL, B::TW, B::TWR, B::PADSH_COL, false>{});
+      }
+      if constexpr ((B::TW * 8) % 64 == 0) {
+        cn(ColV<8, B::WPE_COL, B::TW, B::TWR, B::PADSH_COL, true>{});
+        cn(ColV<8, B::WPE_COL, B::TW, B::TWR, B::PADSH_COL, false>{});
+      }
+    }
     // one padding slot per 4 / 16 / 32 elements (per 4: exchange offsets stay
     // constants where the strides are multiples of 4)
     col_variant<ColV<16, 1, 0, B::TWR, 2>>(b, 16, 1, bytes);
