@@ -164,6 +164,17 @@ def choose(results):
             cur = next((r["ms"] for r in rs if r["pass"] == "col"), None)
             if cur and cnt["ms"] < 0.98 * cur:
                 k["ntc"] = False
+        # tile width x cache policy x layout of the pair of passes (the
+        # staged batches' layout decides which row time goes with it)
+        cwn = [r for r in rs if r["pass"] == "col_cw_nt" and r["ms"] < 1e20]
+        if cwn and lay:
+            cur = (lay["row_panel_ms"] + lay["col_panel_ms"]) if k.get("panel") else (lay["row_ms"] + lay["col_ms"])
+            cand = [(lay["row_ms"] + r["ms"], r, False) for r in cwn]
+            cand += [(lay["row_panel_ms"] + r["panel_ms"], r, True) for r in cwn if r["panel_ms"] < 1e20]
+            t, r, pan = min(cand, key=lambda c: c[0])
+            if t < 0.98 * cur:
+                k["cw"], k["panel"] = r["cw"], pan
+                k["ntc"] = bool(r["nt"])
         plans = [r for r in rs if r["pass"] == "plan"]
         if plans:  # the row and the column transforms each take their fastest plan
             dname = ",".join(map(str, default_plan(n)))

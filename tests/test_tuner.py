@@ -36,6 +36,19 @@ def test_choices_per_length():
     assert ch[2048] == {}
 
 
+def test_tile_width_policy_and_layout_chosen_together():
+    """768-like timings: the pair of passes
+    (row in the staged layout + column) decides width, cache policy and
+    layout together."""
+    res = _lines(768, panel=(0.386, 0.422, 0.674, 0.671)) + [
+        {"n": 768, "pass": "col_cw_nt", "cw": 16, "nt": 1, "ms": 0.674, "panel_ms": 0.670},
+        {"n": 768, "pass": "col_cw_nt", "cw": 16, "nt": 0, "ms": 0.666, "panel_ms": 0.625},
+        {"n": 768, "pass": "col_cw_nt", "cw": 8, "nt": 1, "ms": 0.739, "panel_ms": 0.572},
+        {"n": 768, "pass": "col_cw_nt", "cw": 8, "nt": 0, "ms": 0.691, "panel_ms": 0.596}]
+    k = T.choose(res)[768]
+    assert k["cw"] == 8 and k["panel"] is True and k["ntc"] is True
+
+
 def test_emit_writes_chosen_fields(tmp_path):
     res = _lines(480, flat_ms=0.9, panel=(1.0, 0.95, 1.0, 0.9)) + _lines(1024, x16=0.706, x8=0.597)
     path = tmp_path / "tuned.h"
