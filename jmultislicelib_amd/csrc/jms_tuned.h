@@ -104,7 +104,7 @@ template <> struct Tuned<128> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
-  static constexpr bool PANEL = false;
+  static constexpr bool PANEL = true;
   static constexpr int TR = 0;
   static constexpr bool NTC = true;
 };
@@ -128,7 +128,7 @@ template <> struct Tuned<192> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
-  static constexpr bool PANEL = false;
+  static constexpr bool PANEL = true;
   static constexpr int TR = 0;
   static constexpr bool NTC = true;
 };
@@ -140,7 +140,7 @@ template <> struct Tuned<256> {
   static constexpr int TW = 0;
   static constexpr int PADC = 0;
   static constexpr int PADR = 0;
-  static constexpr bool PANEL = false;
+  static constexpr bool PANEL = true;
   static constexpr int TR = 0;
   static constexpr bool NTC = true;
 };

@@ -49,7 +49,7 @@ def _engine(sp, dx, dy, w, var, batch=None):
     return eng
 
 
-@pytest.mark.parametrize("edge", [360, 512, 625, 768, 1024])
+@pytest.mark.parametrize("edge", [128, 192, 256, 360, 512, 625, 768, 1024])
 def test_group_equals_separate_runs_and_oracle(edge):
     jobs = _jobs(edge)
     # member 2 gets a batch of two waves: it runs over several rounds
